@@ -1,0 +1,53 @@
+"""Build the tracker's native libraries in-tree.
+
+librecon_amd.so   HIP kernels + C ABI (hipcc, gfx950)
+librecon_synth.so seeded synthetic RGB-D generator (g++, OpenMP)
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROOT = os.path.dirname(HERE)
+
+HIP_SOURCES = ["rt_kernels.hip", "rt_tracker.hip", "rt_batch.hip"]
+HIPCC_FLAGS = [
+    "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
+    # every FP32 value is the rounded result of the written operation sequence
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wno-unused-result",
+]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("build failed: " + " ".join(cmd[:3]))
+    return out
+
+
+def build_hip():
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
+    out = os.path.join(HERE, "librecon_amd.so")
+    hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+    _run([hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", out] + srcs)
+    return out
+
+
+def build_synth():
+    out = os.path.join(HERE, "librecon_synth.so")
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-o", out, os.path.join(CSRC, "synth.cpp")])
+    return out
+
+
+def build_all():
+    return [build_synth(), build_hip()]
+
+
+if __name__ == "__main__":
+    for p in build_all():
+        print(p)

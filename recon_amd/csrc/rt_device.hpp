@@ -1,0 +1,138 @@
+// Device-side geometry and sampling helpers of the tracker kernels.
+//
+// Expressions keep the operand order of the reference's device helpers
+// (tracker/kernels/kernelUtils.h) and the library is compiled with
+// -ffp-contract=off, so every FP32 value is the correctly rounded result of the
+// same sequence of operations as in the reference source.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rt {
+
+// calibration block offsets (tracker/calib/calib.h:20-36)
+enum {
+    KR = 0, IKR = 9, KCR = 18, TLR = 26, C1 = 42, KL = 43, IKL = 52, C0 = 61, MIND = 62, MAXD = 63, TRL = 64,
+    CALIB = 96
+};
+
+struct f3 { float x, y, z; };
+struct f2 { float x, y; };
+
+// matrixMultVec4 (kernelUtils.h:242-247)
+__device__ __forceinline__ f3 mul4(const float* T, f3 v) {
+    f3 r;
+    r.x = T[0] * v.x + T[1] * v.y + T[2] * v.z + T[3];
+    r.y = T[4] * v.x + T[5] * v.y + T[6] * v.z + T[7];
+    r.z = T[8] * v.x + T[9] * v.y + T[10] * v.z + T[11];
+    return r;
+}
+// matrixRot4 (kernelUtils.h:249-254)
+__device__ __forceinline__ f3 rot4(const float* T, f3 v) {
+    f3 r;
+    r.x = T[0] * v.x + T[1] * v.y + T[2] * v.z;
+    r.y = T[4] * v.x + T[5] * v.y + T[6] * v.z;
+    r.z = T[8] * v.x + T[9] * v.y + T[10] * v.z;
+    return r;
+}
+// matrixMultVec3 (kernelUtils.h:235-240)
+__device__ __forceinline__ f3 mul3(const float* K, f3 v) {
+    f3 r;
+    r.x = K[0] * v.x + K[1] * v.y + K[2] * v.z;
+    r.y = K[3] * v.x + K[4] * v.y + K[5] * v.z;
+    r.z = K[6] * v.x + K[7] * v.y + K[8] * v.z;
+    return r;
+}
+// distortPoint (kernelUtils.h:394-408)
+__device__ __forceinline__ f2 distort(f2 pu, const float* kc, const float* K) {
+    float dx = pu.x * pu.x, dy = pu.y * pu.y;
+    float r2 = dx + dy, r4 = r2 * r2, r6 = r4 * r2;
+    float rd = 1.0f + kc[0] * r2 + kc[1] * r4 + kc[4] * r6;
+    f2 pd;
+    pd.x = K[0] * pu.x * rd + K[2];
+    pd.y = K[4] * pu.y * rd + K[5];
+    return pd;
+}
+// inBounds (kernelUtils.h:341-344)
+__device__ __forceinline__ bool in_bounds(f2 p, int w, int h) {
+    return p.x >= 10.0f && p.x < (float)(w - 10) && p.y >= 10.0f && p.y < (float)(h - 10);
+}
+// bilinearInterpolation(xdi,ydi,fx,fy,...) (kernelUtils.h:267-272)
+__device__ __forceinline__ float blend(float v0, float v1, float v2, float v3, float fx, float fy) {
+    return (1.0f - fx) * (1.0f - fy) * v0 + fx * (1.0f - fy) * v1 + (1.0f - fx) * fy * v2 + fx * fy * v3;
+}
+__device__ __forceinline__ float bilerp(const float* __restrict__ img, int width, int xdi, int ydi, float fx, float fy) {
+    int o = xdi + ydi * width;
+    return blend(img[o], img[o + 1], img[o + width], img[o + width + 1], fx, fy);
+}
+// The same with every tap clamped into the image: used where the reference's
+// taps can leave the image (the coarsest attribute level near the border).
+__device__ __forceinline__ float bilerp_clamped(const float* __restrict__ img, int w, int h, int xdi, int ydi,
+                                                float fx, float fy) {
+    int x0 = min(max(xdi, 0), w - 1), x1 = min(max(xdi + 1, 0), w - 1);
+    int y0 = min(max(ydi, 0), h - 1), y1 = min(max(ydi + 1, 0), h - 1);
+    return blend(img[x0 + y0 * w], img[x1 + y0 * w], img[x0 + y1 * w], img[x1 + y1 * w], fx, fy);
+}
+// convertZMapToXYZKernel (rgb2gray.cu:570-593): back-projection with KL,
+// valid iff -max < z < -min. Returns validity; p = 0 when invalid.
+__device__ __forceinline__ bool backproject(float z, int xi, int yi, const float* c, f3& p) {
+    float mn = c[MIND], mx = c[MAXD];
+    if (z < -mn && z > -mx) {
+        float fx = c[KL], fy = c[KL + 4], cx = c[KL + 2], cy = c[KL + 5];
+        p.x = ((float)xi - cx) * z / fx;
+        p.y = ((float)yi - cy) * z / fy;
+        p.z = z;
+        return true;
+    }
+    p.x = 0.0f; p.y = 0.0f; p.z = 0.0f;
+    return false;
+}
+
+// Level-0 photometric features of one keyframe pixel (xyz2CloudKernel,
+// d2z.cu:235-331): point, gray and central-difference gradient. A pixel without
+// a valid projection gets the reference's empty record: point (0,0,-maxDist),
+// zero gray and gradient.
+struct PixFeat { f3 p; float c1, gx, gy, gmag; bool ok; };
+__device__ __forceinline__ PixFeat pixel_features(const float* __restrict__ z, const float* __restrict__ g0, int W,
+                                                  int H, int xi, int yi, const float* c) {
+    PixFeat f;
+    f3 p3;
+    backproject(z[xi + yi * W], xi, yi, c, p3);
+    if (p3.z < 0.0f) {
+        f3 r3 = mul4(c + TLR, p3);
+        f2 pu{r3.x / r3.z, r3.y / r3.z};
+        f2 p = distort(pu, c + KCR, c + KR);
+        if (in_bounds(p, W, H)) {
+            int xdi = (int)p.x, ydi = (int)p.y;
+            float fx = p.x - (float)xdi, fy = p.y - (float)ydi;
+            float c1 = bilerp(g0, W, xdi, ydi, fx, fy);
+            float cN = bilerp(g0, W, (int)p.x, (int)(p.y - 1.0f), fx, fy);
+            float cE = bilerp(g0, W, (int)(p.x + 1.0f), (int)p.y, fx, fy);
+            float cW = bilerp(g0, W, (int)(p.x - 1.0f), (int)p.y, fx, fy);
+            float cS = bilerp(g0, W, (int)p.x, (int)(p.y + 1.0f), fx, fy);
+            f.gx = (cE - cW) / 2.0f;
+            f.gy = (cS - cN) / 2.0f;
+            f.gmag = fminf(fabsf(f.gy) + fabsf(f.gx), 1.0f);
+            f.c1 = c1;
+            f.p = p3;
+            f.ok = true;
+            return f;
+        }
+    }
+    f.p = f3{0.0f, 0.0f, -c[MAXD]};
+    f.c1 = 0.0f; f.gx = 0.0f; f.gy = 0.0f; f.gmag = 0.0f; f.ok = false;
+    return f;
+}
+
+// gradient-magnitude bin (gpu_hist.cu:135, filterIndex256.cu:126), fused multiply-add
+__device__ __forceinline__ unsigned grad_bin(float g) { return (unsigned)__builtin_fmaf(g, 255.0f, 0.5f); }
+
+// wave64 helpers
+__device__ __forceinline__ float wave_sum(float v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+
+}  // namespace rt

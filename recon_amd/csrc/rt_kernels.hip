@@ -1,0 +1,1122 @@
+// HIP kernels of the single-pair tracker path (gfx950, wave64).
+//
+// Keyframe preparation: one fused convert+pyramid pass, a per-pixel feature
+// pass that also builds the 256-bin gradient histogram in LDS, a three-kernel
+// ordered selection (segment counts, one-workgroup scan with the reference's
+// trim rule, ordered scatter), and one gather pass that writes each selected
+// point's position and its per-level Jacobian records.
+// Gauss-Newton iteration: a residual pass (bilinear warp, zWeight, MAD
+// histogram in LDS) and a system pass (robust weights and the 27 unique
+// normal-equation terms: FP32 per workgroup, combined across workgroups in FP64
+// by the last workgroup to finish, which also runs the FP64 solve and the SE(3)
+// update). The geometric side adds an ordered compaction of the valid ICP
+// pixels, three Student-t variance rounds and its own system pass.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "rt_device.hpp"
+#include "rt_kernels.hpp"
+
+namespace rt {
+
+// ------------------------------------------------------------------ tickets
+enum { TK_PHOTO = 0, TK_ICP = 1, TK_ST = 2, TK_FUSE = 3 };
+
+__device__ __forceinline__ bool last_group(unsigned* ticket) {
+    __shared__ bool is_last;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (t == gridDim.x * gridDim.y - 1);
+    }
+    __syncthreads();
+    if (is_last) __threadfence();
+    return is_last;
+}
+__device__ __forceinline__ void reset_ticket(unsigned* ticket) {
+    __hip_atomic_store(ticket, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ------------------------------------------------------------------ A1 + A2
+// rgbTex2GrayHdrZKernel (rgb2gray.cu:64-76) / u8-u16 conversion, fused with
+// fastDownSampleHdrKernel (downSample2Cuda.cu:35-42) for all levels. Each
+// thread owns a 4x4 level-0 block (its 2x2 level-1 block and one level-2
+// pixel); level 3 comes from the workgroup's 16x16 level-2 tile in LDS.
+// src: 0 = u8 gray + u16 depth, 1 = float4 RGBD (bottom-up), 2 = level 0
+// already in place (rebuild levels >= 1 only).
+template <int SRC>
+__global__ __launch_bounds__(256) void k_convert_pyramid(const uint8_t* __restrict__ g8,
+                                                         const uint16_t* __restrict__ d16,
+                                                         const float4* __restrict__ rgbd, int W, int H, int L,
+                                                         const float* __restrict__ c, float* __restrict__ g0,
+                                                         float* __restrict__ g1, float* __restrict__ g2,
+                                                         float* __restrict__ g3, float* __restrict__ z,
+                                                         unsigned* zero_a, int na, unsigned* zero_b, int nb) {
+    __shared__ float l2tile[16][17];
+    int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+    int x0 = (blockIdx.x * 16 + tx) * 4, y0 = (blockIdx.y * 16 + ty) * 4;
+    bool inside = x0 < W && y0 < H;
+    float l2v = 0.0f;
+    if (inside) {
+        float v[4][4];
+        float mn = c[MIND], mx = c[MAXD];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            int y = y0 + r;
+            size_t o = (size_t)y * W + x0;
+            if (SRC == 0) {
+                uchar4 gg = *reinterpret_cast<const uchar4*>(g8 + o);
+                ushort4 dd = *reinterpret_cast<const ushort4*>(d16 + o);
+                v[r][0] = (float)gg.x / 255.0f; v[r][1] = (float)gg.y / 255.0f;
+                v[r][2] = (float)gg.z / 255.0f; v[r][3] = (float)gg.w / 255.0f;
+                *reinterpret_cast<float4*>(g0 + o) = make_float4(v[r][0], v[r][1], v[r][2], v[r][3]);
+                *reinterpret_cast<float4*>(z + o) =
+                    make_float4(-(float)dd.x, -(float)dd.y, -(float)dd.z, -(float)dd.w);
+            } else if (SRC == 1) {
+                float zz[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    float4 p = rgbd[(size_t)(x0 + q) + (size_t)(H - 1 - y) * W];
+                    v[r][q] = 0.3f * p.x + 0.59f * p.y + 0.11f * p.z;
+                    zz[q] = -(p.w * (mx - mn) + mn);
+                }
+                *reinterpret_cast<float4*>(g0 + o) = make_float4(v[r][0], v[r][1], v[r][2], v[r][3]);
+                *reinterpret_cast<float4*>(z + o) = make_float4(zz[0], zz[1], zz[2], zz[3]);
+            } else {
+                float4 gg = *reinterpret_cast<const float4*>(g0 + o);
+                v[r][0] = gg.x; v[r][1] = gg.y; v[r][2] = gg.z; v[r][3] = gg.w;
+            }
+        }
+        if (L > 1) {
+            float l1[2][2];
+            int W1 = W >> 1;
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    l1[i][j] = (v[2 * i][2 * j] + v[2 * i][2 * j + 1] + v[2 * i + 1][2 * j] + v[2 * i + 1][2 * j + 1]) / 4.0f;
+                *reinterpret_cast<float2*>(g1 + (size_t)(y0 / 2 + i) * W1 + x0 / 2) = make_float2(l1[i][0], l1[i][1]);
+            }
+            if (L > 2) {
+                l2v = (l1[0][0] + l1[0][1] + l1[1][0] + l1[1][1]) / 4.0f;
+                g2[(size_t)(y0 / 4) * (W >> 2) + x0 / 4] = l2v;
+            }
+        }
+    }
+    if (L > 3) {
+        l2tile[ty][tx] = l2v;
+        __syncthreads();
+        if (inside && (tx & 1) == 0 && (ty & 1) == 0) {
+            float l3 = (l2tile[ty][tx] + l2tile[ty][tx + 1] + l2tile[ty + 1][tx] + l2tile[ty + 1][tx + 1]) / 4.0f;
+            g3[(size_t)(y0 / 8) * (W >> 3) + x0 / 8] = l3;
+        }
+    }
+    if (blockIdx.x == 0 && blockIdx.y == 0) {
+        for (int i = threadIdx.x; i < na; i += 256) zero_a[i] = 0u;
+        for (int i = threadIdx.x; i < nb; i += 256) zero_b[i] = 0u;
+    }
+}
+
+void launch_convert_pyramid(const Dev& d, const Pyr& p, const uint8_t* g8, const uint16_t* d16, const float* rgbd,
+                            bool zero_madhist, bool zero_ghist, hipStream_t s) {
+    dim3 grid((d.W / 4 + 15) / 16, (d.H / 4 + 15) / 16);
+    unsigned* za = zero_madhist ? d.madhist : nullptr;
+    unsigned* zb = zero_ghist ? d.ghist : nullptr;
+    int na = zero_madhist ? 256 : 0, nb = zero_ghist ? 256 : 0;
+    if (g8)
+        k_convert_pyramid<0><<<grid, 256, 0, s>>>(g8, d16, nullptr, d.W, d.H, d.L, d.calib, p.gray[0], p.gray[1],
+                                                  p.gray[2], p.gray[3], p.z, za, na, zb, nb);
+    else if (rgbd)
+        k_convert_pyramid<1><<<grid, 256, 0, s>>>(nullptr, nullptr, reinterpret_cast<const float4*>(rgbd), d.W, d.H,
+                                                  d.L, d.calib, p.gray[0], p.gray[1], p.gray[2], p.gray[3], p.z, za, na,
+                                                  zb, nb);
+    else
+        k_convert_pyramid<2><<<grid, 256, 0, s>>>(nullptr, nullptr, nullptr, d.W, d.H, d.L, d.calib, p.gray[0],
+                                                  p.gray[1], p.gray[2], p.gray[3], p.z, za, na, zb, nb);
+}
+
+// Clears the relative pose: TrelDev and, for sequences, the device copy of the
+// host relative pose (PhotoTracker::prepareReference, phototracker.cpp:363-369).
+__global__ void k_reset_pose(float* T, double* pose) {
+    int i = threadIdx.x;
+    if (i < 16) {
+        T[i] = (i % 5 == 0) ? 1.0f : 0.0f;
+        if (pose) pose[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+}
+void launch_reset_pose(float* T, double* pose, hipStream_t s) { k_reset_pose<<<1, 64, 0, s>>>(T, pose); }
+
+__global__ void k_fill(float* p, float v, int n) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill(float* p, float v, int n, hipStream_t s) { k_fill<<<(n + 255) / 256, 256, 0, s>>>(p, v, n); }
+
+// ------------------------------------------------------------------ A3-A6
+// xyz2CloudKernel (d2z.cu:235-331) reduced to the gradient magnitude the
+// selection needs, plus the 256-bin histogram (cuda_hist.cu:100-152) built in
+// LDS and added to the global histogram once per workgroup.
+__global__ __launch_bounds__(256) void k_ref_features(const float* __restrict__ z, const float* __restrict__ g0, int W,
+                                                      int H, const float* __restrict__ c, float* __restrict__ gmag,
+                                                      unsigned* __restrict__ ghist) {
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x < W && y < H) {
+        PixFeat f = pixel_features(z, g0, W, H, x, y, c);
+        gmag[x + (size_t)y * W] = f.gmag;
+        atomicAdd(&h[grad_bin(f.gmag)], 1u);
+    }
+    __syncthreads();
+    unsigned v = h[threadIdx.x];
+    if (v) atomicAdd(&ghist[threadIdx.x], v);
+}
+
+// ------------------------------------------------------------------ A7
+// seekThreshold256Kernel2 (filterIndex256.cu:26-46): scanning the bins from the
+// top, the threshold is the first bin whose running count exceeds N.
+__device__ void find_threshold(const unsigned* __restrict__ ghist, int N, int& thr, int& excess) {
+    __shared__ int sc[256];
+    __shared__ int first;
+    int t = threadIdx.x;
+    if (t < 256) sc[t] = (int)ghist[255 - t];
+    if (t == 0) first = 256;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int v = (t < 256 && t >= off) ? sc[t - off] : 0;
+        __syncthreads();
+        if (t < 256) sc[t] += v;
+        __syncthreads();
+    }
+    if (t < 256 && sc[t] > N && (t == 0 || sc[t - 1] <= N)) first = t;
+    __syncthreads();
+    if (first < 256) { thr = 255 - first; excess = sc[first] - N; }
+    else { thr = 0; excess = sc[255] - N; }
+    __syncthreads();
+}
+
+// filterIndexKernel4 (filterIndex256.cu:110-169), pass 1: per-segment counts of
+// bins above and equal to the threshold. 512 segments of n/512 pixels.
+__global__ __launch_bounds__(256) void k_select_count(const float* __restrict__ gmag, int npix, const unsigned* ghist,
+                                                      int N, int* seg) {
+    int thr, excess;
+    find_threshold(ghist, N, thr, excess);
+    __shared__ int cA, cE;
+    if (threadIdx.x == 0) { cA = 0; cE = 0; }
+    __syncthreads();
+    int P = npix / 512, s = blockIdx.x, b0 = s * P;
+    int a = 0, e = 0;
+    for (int i = b0 + threadIdx.x; i < b0 + P; i += 256) {
+        unsigned bn = grad_bin(gmag[i]);
+        a += bn > (unsigned)thr;
+        e += bn == (unsigned)thr;
+    }
+    a = (int)wave_sum((float)a);
+    e = (int)wave_sum((float)e);
+    if (lane_id() == 0) { atomicAdd(&cA, a); atomicAdd(&cE, e); }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        seg[s] = cA;
+        seg[512 + s] = cE;
+        if (s == 0) { seg[2560] = thr; seg[2561] = excess; }
+    }
+}
+
+// pass 2 (one workgroup of 512): segment start offsets; the trim loop of
+// filterIndex256.cu:141-152 run as written.
+__global__ __launch_bounds__(512) void k_select_scan(int* seg) {
+    __shared__ int a[512], m[512], sa[512], se[512];
+    int t = threadIdx.x;
+    a[t] = seg[t];
+    m[t] = seg[512 + t];
+    __syncthreads();
+    if (t == 0) {
+        int tooMuch = seg[2561];
+        for (int i = 1; i < 512; i++) {
+            if (tooMuch > 0 && m[i - 1] > 0) {
+                m[i - 1] -= tooMuch;
+                if (m[i - 1] < 0) { tooMuch += m[i - 1]; m[i - 1] = 0; }
+                else tooMuch = 0;
+            }
+        }
+    }
+    sa[t] = a[t];
+    se[t] = m[t];
+    __syncthreads();
+    for (int off = 1; off < 512; off <<= 1) {
+        int va = t >= off ? sa[t - off] : 0, ve = t >= off ? se[t - off] : 0;
+        __syncthreads();
+        sa[t] += va; se[t] += ve;
+        __syncthreads();
+    }
+    int totalA = sa[511];
+    seg[1024 + t] = sa[t] - a[t];
+    seg[1536 + t] = totalA + se[t] - m[t];
+    seg[2048 + t] = m[t];
+    if (t == 0) { seg[2562] = totalA; seg[2563] = totalA + se[511]; }
+}
+
+// pass 3: ordered scatter of each segment's above- and equal-bin indices.
+__global__ __launch_bounds__(256) void k_select_scatter(const float* __restrict__ gmag, int npix, int N,
+                                                        const int* __restrict__ seg, int* __restrict__ sel) {
+    __shared__ int wa[4], we[4];
+    __shared__ int runA, runE;
+    int s = blockIdx.x, t = threadIdx.x, wave = t >> 6, lane = lane_id();
+    unsigned thr = (unsigned)seg[2560];
+    int startA = seg[1024 + s], startE = seg[1536 + s], mE = seg[2048 + s];
+    int P = npix / 512, b0 = s * P;
+    if (t == 0) { runA = 0; runE = 0; }
+    __syncthreads();
+    uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    for (int base = b0; base < b0 + P; base += 256) {
+        int i = base + t;
+        bool in = i < b0 + P;
+        unsigned bn = in ? grad_bin(gmag[i]) : 0u;
+        bool isA = in && bn > thr, isE = in && bn == thr;
+        uint64_t ma = __ballot(isA), me = __ballot(isE);
+        if (lane == 0) { wa[wave] = __popcll(ma); we[wave] = __popcll(me); }
+        __syncthreads();
+        int pa = 0, pe = 0;
+        for (int q = 0; q < wave; q++) { pa += wa[q]; pe += we[q]; }
+        int rA = runA + pa + __popcll(ma & lt);
+        int rE = runE + pe + __popcll(me & lt);
+        if (isA) { int pos = startA + rA; if (pos < N) sel[pos] = i; }
+        if (isE && rE < mE) { int pos = startE + rE; if (pos < N) sel[pos] = i; }
+        __syncthreads();
+        if (t == 0) { runA += wa[0] + wa[1] + wa[2] + wa[3]; runE += we[0] + we[1] + we[2] + we[3]; }
+        __syncthreads();
+    }
+    if (s == 0) {
+        int total = seg[2563];
+        for (int k = (total > 0 ? total : 0) + t; k < N; k += 256) sel[k] = 0;
+    }
+}
+
+// ------------------------------------------------------------------ A8 + A9
+// addVertexAttributesWithoutNormalsKernel3/4 (d2z.cu:700-936) and
+// precomputeJacobian4Kernel (warp.cu:502-645) for the selected points. Per
+// point and level the record holds {J0,J1,J2,J3} and {J4,J5,I_ref,0}.
+__global__ __launch_bounds__(256) void k_ref_points(const float* __restrict__ z, const float* __restrict__ g0,
+                                                    const float* __restrict__ g1, const float* __restrict__ g2,
+                                                    const float* __restrict__ g3, int W, int H, int L, int N,
+                                                    const float* __restrict__ c, float scale_in,
+                                                    const int* __restrict__ sel, float4* __restrict__ pts,
+                                                    float4* __restrict__ j0, float4* __restrict__ j1,
+                                                    float4* __restrict__ j2, float4* __restrict__ j3) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    int idx = sel[i];
+    int xi = idx % W, yi = idx / W;
+    PixFeat f = pixel_features(z, g0, W, H, xi, yi, c);
+    float gx[4] = {f.gx, 0.0f, 0.0f, 0.0f}, gy[4] = {f.gy, 0.0f, 0.0f, 0.0f}, col[4] = {f.c1, 0.0f, 0.0f, 0.0f};
+    const float* lv[4] = {g0, g1, g2, g3};
+    {
+        f3 r3 = mul4(c + TLR, f.p);
+        f2 pu{r3.x / r3.z, r3.y / r3.z};
+        f2 p1 = distort(pu, c + KCR, c + KR);
+        if (in_bounds(p1, W, H)) {
+            f2 q = p1;
+            for (int l = 1; l < L; l++) {
+                q.x = 0.5f * q.x + -0.25f;
+                q.y = 0.5f * q.y + -0.25f;
+                int wl = W >> l, hl = H >> l;
+                int xdi = (int)q.x, ydi = (int)q.y;
+                float fX = q.x - (float)xdi, fY = q.y - (float)ydi;
+                const float* img = lv[l];
+                float cc = bilerp_clamped(img, wl, hl, xdi, ydi, fX, fY);
+                float cN = bilerp_clamped(img, wl, hl, xdi, ydi - 1, fX, fY);
+                float cW = bilerp_clamped(img, wl, hl, xdi - 1, ydi, fX, fY);
+                float cE = bilerp_clamped(img, wl, hl, xdi + 1, ydi, fX, fY);
+                float cS = bilerp_clamped(img, wl, hl, xdi, ydi + 1, fX, fY);
+                gx[l] = (cE - cW) / 2.0f;
+                gy[l] = (cS - cN) / 2.0f;
+                col[l] = cc;
+            }
+        }
+    }
+    // Jacobian (warp.cu:502-645)
+    const float* K = c + KR;
+    const float* T = c + TLR;
+    const float* kc = c + KCR;
+    f3 r3 = mul4(T, f.p);
+    r3.x *= scale_in; r3.y *= scale_in; r3.z *= scale_in;
+    float dN0 = 1.0f / r3.z, dN2 = -r3.x / (r3.z * r3.z), dN4 = 1.0f / r3.z, dN5 = -r3.y / (r3.z * r3.z);
+    float x = r3.x / r3.z, y = r3.y / r3.z;
+    float x2 = x * x, y2 = y * y, x4 = x2 * x2, y4 = y2 * y2, r2 = x2 + y2, r4 = r2 * r2;
+    float k0 = kc[0], k1 = kc[1], k4 = kc[4];
+    float dD0 = 1.0f + k0 * (3.0f * x2 + y2) + k1 * (5.0f * x4 + 6.0f * x2 * y2 + y4) + k4 * r4 * (7.0f * x2 + y2);
+    float dD1 = k0 * 2.0f * x * y + k1 * 4.0f * x * y * r2 + k4 * 6.0f * x * y * r4;
+    float dD2 = k0 * 2.0f * y * x + k1 * 4.0f * x * y * r2 + k4 * 6.0f * x * y * r4;
+    float dD3 = 1.0f + k0 * (3.0f * x2 + y2) + k1 * (5.0f * x4 + 6.0f * x2 * y2 + y4) + k4 * r4 * (7.0f * y2 + x2);
+    f3 gens[6] = {{0.0f, -r3.z, r3.y}, {r3.z, 0.0f, -r3.x}, {-r3.y, r3.x, 0.0f},
+                  {1.0f, 0.0f, 0.0f},  {0.0f, 1.0f, 0.0f},  {0.0f, 0.0f, 1.0f}};
+    float J[4][6];
+#pragma unroll
+    for (int k = 0; k < 6; k++) {
+        f3 dr3 = rot4(T, gens[k]);
+        float px = dN0 * dr3.x + 0.0f * dr3.y + dN2 * dr3.z;
+        float py = 0.0f * dr3.x + dN4 * dr3.y + dN5 * dr3.z;
+        f3 dd2{dD0 * px + dD1 * py, dD2 * px + dD3 * py, 0.0f};
+        f3 dp2 = mul3(K, dd2);
+        J[0][k] = dp2.x * gx[0] + dp2.y * gy[0];
+#pragma unroll
+        for (int l = 1; l < 4; l++) {
+            float sc = (float)(1 << l);
+            J[l][k] = (dp2.x / sc) * gx[l] + (dp2.y / sc) * gy[l];
+        }
+    }
+    pts[i] = make_float4(f.p.x, f.p.y, f.p.z, 0.0f);
+    float4* jl[4] = {j0, j1, j2, j3};
+    for (int l = 0; l < L; l++) {
+        jl[l][2 * i] = make_float4(J[l][0], J[l][1], J[l][2], J[l][3]);
+        jl[l][2 * i + 1] = make_float4(J[l][4], J[l][5], col[l], 0.0f);
+    }
+}
+
+void launch_prepare_photo_reference(const Dev& d, hipStream_t s) {
+    dim3 g16((d.W + 15) / 16, (d.H + 15) / 16);
+    k_ref_features<<<g16, 256, 0, s>>>(d.ref.z, d.ref.gray[0], d.W, d.H, d.calib, d.gmag, d.ghist);
+    k_select_count<<<512, 256, 0, s>>>(d.gmag, d.W * d.H, d.ghist, d.N, d.seg);
+    k_select_scan<<<1, 512, 0, s>>>(d.seg);
+    k_select_scatter<<<512, 256, 0, s>>>(d.gmag, d.W * d.H, d.N, d.seg, d.sel);
+    k_ref_points<<<(d.N + 255) / 256, 256, 0, s>>>(d.ref.z, d.ref.gray[0], d.ref.gray[1], d.ref.gray[2], d.ref.gray[3],
+                                                    d.W, d.H, d.L, d.N, d.calib, 1e-3f, d.sel, d.pts, d.jac[0],
+                                                    d.jac[1], d.jac[2], d.jac[3]);
+}
+
+// ------------------------------------------------------------------ solve
+// conjugateGradientKernel (warp.cu:926-975) and a textbook FP64 Cholesky.
+__device__ void solve_cg(const double* A, const double* bb, double* x) {
+    double r[6], dir[6];
+    const double tol = 1e-8;
+    for (int i = 0; i < 6; i++) { x[i] = 0.0; r[i] = bb[i]; dir[i] = bb[i]; }
+    for (int step = 0; step < 6; step++) {
+        double Ad[6];
+        for (int j = 0; j < 6; j++) { Ad[j] = 0.0; for (int k = 0; k < 6; k++) Ad[j] += A[j * 6 + k] * dir[k]; }
+        double rr = 0.0, Add = 0.0;
+        for (int i = 0; i < 6; i++) rr += r[i] * r[i];
+        for (int i = 0; i < 6; i++) Add += Ad[i] * dir[i];
+        double dv = Add < 0 ? -Add : Add;
+        double st = 0.0;
+        if (dv > tol) st = rr / Add;
+        for (int i = 0; i < 6; i++) r[i] -= st * Ad[i];
+        double rr2 = 0.0;
+        for (int i = 0; i < 6; i++) rr2 += r[i] * r[i];
+        for (int i = 0; i < 6; i++) x[i] += st * dir[i];
+        if (sqrt(rr2) < tol) return;
+        double beta = rr2 / rr;
+        for (int i = 0; i < 6; i++) dir[i] = r[i] + beta * dir[i];
+    }
+}
+__device__ bool solve_cholesky(const double* A, const double* b, double* x) {
+    double Lm[36];
+    for (int i = 0; i < 36; i++) Lm[i] = 0.0;
+    for (int j = 0; j < 6; j++) {
+        double s = A[j * 6 + j];
+        for (int k = 0; k < j; k++) s -= Lm[j * 6 + k] * Lm[j * 6 + k];
+        if (!(s > 0.0)) { for (int i = 0; i < 6; i++) x[i] = 0.0; return false; }
+        double dg = sqrt(s);
+        Lm[j * 6 + j] = dg;
+        for (int i = j + 1; i < 6; i++) {
+            double t = A[i * 6 + j];
+            for (int k = 0; k < j; k++) t -= Lm[i * 6 + k] * Lm[j * 6 + k];
+            Lm[i * 6 + j] = t / dg;
+        }
+    }
+    double y[6];
+    for (int i = 0; i < 6; i++) {
+        double t = b[i];
+        for (int k = 0; k < i; k++) t -= Lm[i * 6 + k] * y[k];
+        y[i] = t / Lm[i * 6 + i];
+    }
+    for (int i = 5; i >= 0; i--) {
+        double t = y[i];
+        for (int k = i + 1; k < 6; k++) t -= Lm[k * 6 + i] * x[k];
+        x[i] = t / Lm[i * 6 + i];
+    }
+    return true;
+}
+// normalizeMat4RealSafe (expmCuda.cu:1957-1980): column lengths taken before
+// the cross products, with single-precision square roots.
+__device__ void normalize_mat4_safe(double* T) {
+    double u[3] = {T[0], T[4], T[8]}, v[3] = {T[1], T[5], T[9]}, w[3] = {T[2], T[6], T[10]};
+    double lenU = (double)sqrtf((float)(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    double lenW = (double)sqrtf((float)(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]));
+    if (lenW > 1e-8f) { w[0] /= lenW; w[1] /= lenW; w[2] /= lenW; }
+    else { w[0] = 0; w[1] = 0; w[2] = 1; }
+    u[0] = v[1] * w[2] - v[2] * w[1];
+    u[1] = -(v[0] * w[2] - v[2] * w[0]);
+    u[2] = v[0] * w[1] - v[1] * w[0];
+    if (lenU > 1e-8f) { u[0] /= lenU; u[1] /= lenU; u[2] /= lenU; }
+    else { u[0] = 1; u[1] = 0; u[2] = 0; }
+    v[0] = w[1] * u[2] - w[2] * u[1];
+    v[1] = -(w[0] * u[2] - w[2] * u[0]);
+    v[2] = w[0] * u[1] - w[1] * u[0];
+    T[0] = u[0]; T[4] = u[1]; T[8] = u[2];
+    T[1] = v[0]; T[5] = v[1]; T[9] = v[2];
+    T[2] = w[0]; T[6] = w[1]; T[10] = w[2];
+}
+// Closed-form SE(3) exponential of generateA(x) (warp.cu:910-915).
+__device__ void se3_exp(const double* x, double* E) {
+    double wx = x[0], wy = x[1], wz = x[2];
+    double th2 = wx * wx + wy * wy + wz * wz, th = sqrt(th2);
+    double a, b, cc;
+    if (th < 1e-4) {
+        a = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+        b = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+        cc = 1.0 / 6.0 - th2 / 120.0 + th2 * th2 / 5040.0;
+    } else {
+        double sn = sin(th), co = cos(th);
+        a = sn / th; b = (1.0 - co) / th2; cc = (th - sn) / (th2 * th);
+    }
+    double Wm[9] = {0, -wz, wy, wz, 0, -wx, -wy, wx, 0};
+    double W2[9];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            double s = 0;
+            for (int k = 0; k < 3; k++) s += Wm[i * 3 + k] * Wm[k * 3 + j];
+            W2[i * 3 + j] = s;
+        }
+    double Rm[9], V[9];
+    for (int i = 0; i < 9; i++) {
+        double I = (i % 4 == 0) ? 1.0 : 0.0;
+        Rm[i] = I + a * Wm[i] + b * W2[i];
+        V[i] = I + b * Wm[i] + cc * W2[i];
+    }
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) E[i * 4 + j] = Rm[i * 3 + j];
+        E[i * 4 + 3] = V[i * 3 + 0] * x[3] + V[i * 3 + 1] * x[4] + V[i * 3 + 2] * x[5];
+    }
+    E[12] = 0; E[13] = 0; E[14] = 0; E[15] = 1;
+}
+// expmKernel (expmCuda.cu:1989-2048): T <- normalize(T * normalize(exp(A(x))) with t scaled)
+__device__ void expm_update(const double* x, float* TF, double scale) {
+    double T[16], Ti[16], R[16];
+    for (int i = 0; i < 16; i++) T[i] = TF[i];
+    double mx = 0;
+    for (int i = 0; i < 6; i++) mx = fmax(mx, fabs(x[i]));
+    if (mx != 0) {
+        se3_exp(x, Ti);
+        normalize_mat4_safe(Ti);
+        Ti[3] *= scale; Ti[7] *= scale; Ti[11] *= scale;
+    } else {
+        for (int i = 0; i < 16; i++) Ti[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++)
+            R[i * 4 + j] = T[i * 4 + 0] * Ti[0 * 4 + j] + T[i * 4 + 1] * Ti[1 * 4 + j] + T[i * 4 + 2] * Ti[2 * 4 + j] +
+                           T[i * 4 + 3] * Ti[3 * 4 + j];
+    normalize_mat4_safe(R);
+    for (int i = 0; i < 16; i++) TF[i] = (float)R[i];
+}
+__device__ void unpack27(const double* s, double* A, double* b) {
+    int k = 0;
+    for (int r = 0; r < 6; r++)
+        for (int c2 = r; c2 < 6; c2++) { A[r * 6 + c2] = s[k]; A[c2 * 6 + r] = s[k]; k++; }
+    for (int r = 0; r < 6; r++) b[r] = s[21 + r];
+}
+// estimateIncrement (phototracker.cpp:646-664) with linearFuse (warp.cu:1498-1509)
+__device__ void solve_update(const double* sys, float* trel, const SolveCfg& cfg, int N1, int N2) {
+    double A[36], b[6], x[6];
+    unpack27(cfg.mode == 0 ? sys + 27 : sys, A, b);
+    if (cfg.mode == 2) {
+        double Ag[36], bg[6];
+        unpack27(sys + 27, Ag, bg);
+        double s1 = (1.0 - (double)cfg.lambda) / (double)N1, s2 = (double)cfg.lambda / (double)N2;
+        for (int k = 0; k < 36; k++) A[k] = s1 * A[k] + s2 * Ag[k];
+        for (int k = 0; k < 6; k++) b[k] = s1 * b[k] + s2 * bg[k];
+    }
+    if (cfg.solver == 1) solve_cg(A, b, x);
+    else solve_cholesky(A, b, x);
+    expm_update(x, trel, (double)cfg.scale_out);
+}
+
+// Reduce the 27 per-thread FP32 sums of a 256-thread workgroup (wave tree, then
+// the 4 waves in order) into part[group*27 + k].
+__device__ __forceinline__ void reduce27(float* acc, float* part) {
+    __shared__ float wsum[4][27];
+    int wave = threadIdx.x >> 6, lane = lane_id();
+#pragma unroll
+    for (int k = 0; k < 27; k++) {
+        float v = wave_sum(acc[k]);
+        if (lane == 0) wsum[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 27) {
+        int k = threadIdx.x;
+        part[(size_t)blockIdx.x * 27 + k] = ((wsum[0][k] + wsum[1][k]) + wsum[2][k]) + wsum[3][k];
+    }
+}
+// Last workgroup: FP64 sum over groups in group order.
+__device__ __forceinline__ void combine27(const float* part, int groups, double* out) {
+    if (threadIdx.x < 27) {
+        double s = 0.0;
+        for (int g = 0; g < groups; g++) s += (double)part[(size_t)g * 27 + threadIdx.x];
+        out[threadIdx.x] = s;
+    }
+}
+
+// ------------------------------------------------------------------ A10 + A11
+// interpolateResidualKernel2 (warp.cu:269-332) with the level mapping of
+// interpolateResidual2 (:1160-1218); the MAD histogram of discretizeResidual
+// (histogram64.cu:41-44) is built in LDS.
+__global__ __launch_bounds__(256) void k_photo_res(const float4* __restrict__ pts, const float4* __restrict__ jac,
+                                                   int N, int layer, const float* __restrict__ img, int wl, int hl,
+                                                   const float* __restrict__ zcur, int W, int H,
+                                                   const float* __restrict__ c, const float* __restrict__ trel,
+                                                   int bins, float2* __restrict__ res, unsigned* __restrict__ madhist) {
+    __shared__ unsigned h[256];
+    h[threadIdx.x] = 0u;
+    float T[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) T[k] = trel[k];
+    __syncthreads();
+    float a = 1.0f / (float)(1 << layer);
+    float b = 0.5f * (a - 1.0f);
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+        float4 p4 = pts[i];
+        float iref = jac[2 * i + 1].z;
+        f3 p3{p4.x, p4.y, p4.z};
+        f3 r3ir = mul4(T, p3);
+        f3 r3 = mul4(c + TLR, r3ir);
+        f2 pu{r3.x / r3.z, r3.y / r3.z};
+        f2 p2 = distort(pu, c + KCR, c + KR);
+        f2 p{a * p2.x + b, a * p2.y + b};
+        float ir = 1.0f, zW = 0.0f;
+        // a non-finite or huge projection is outside the view (the conversion
+        // of such a value to int is undefined)
+        bool fin = fabsf(p.x) < 1e9f && fabsf(p.y) < 1e9f;
+        int xdi = fin ? (int)p.x : -1, ydi = fin ? (int)p.y : -1;
+        if (xdi >= 0 && ydi >= 0 && xdi < wl - 1 && ydi < hl - 1) {
+            float fx = p.x - (float)xdi, fy = p.y - (float)ydi;
+            float color = bilerp(img, wl, xdi, ydi, fx, fy);
+            ir = iref - color;
+            f3 rl3 = mul4(c + TRL, r3);
+            rl3.x /= rl3.z; rl3.y /= rl3.z; rl3.z = 1.0f;
+            f3 pl2 = mul3(c + KL, rl3);
+            bool fin2 = fabsf(pl2.x) < 1e9f && fabsf(pl2.y) < 1e9f;
+            int xdi2 = fin2 ? (int)(pl2.x + 0.5f) : -1, ydi2 = fin2 ? (int)(pl2.y + 0.5f) : -1;
+            if (xdi2 >= 0 && ydi2 >= 0 && xdi2 < W && ydi2 < H) {
+                float zc = zcur[xdi2 + (size_t)ydi2 * W];
+                float zerr = zc - r3.z;
+                zerr *= zerr;
+                if (zerr < 10000.0f) zW = 1.0f - zerr / 10000.0f;
+            }
+        }
+        res[i] = make_float2(ir, zW);
+        int bn = (int)((float)(bins - 1) * fabsf(ir));
+        if (bn >= 0 && bn < bins) atomicAdd(&h[bn], 1u);
+    }
+    __syncthreads();
+    if (threadIdx.x < bins) {
+        unsigned v = h[threadIdx.x];
+        if (v) atomicAdd(&madhist[threadIdx.x], v);
+    }
+}
+
+// generateWeights64Kernel (histogram64.cu:79-94) + weightJacobianKernel
+// (warp.cu:770-780) + the 27 dot products (warp.cu:1453-1487).
+__global__ __launch_bounds__(256) void k_photo_sys(const float2* __restrict__ res, const float4* __restrict__ jac,
+                                                   int N, int bins, int robust, unsigned* madhist,
+                                                   float* __restrict__ wts, float* part, double* sys, float* trel,
+                                                   unsigned* ticket, SolveCfg cfg, int solve_here) {
+    __shared__ int hs[256];
+    __shared__ int med;
+    int t = threadIdx.x;
+    hs[t] = t < bins ? (int)madhist[t] : 0;
+    if (t == 0) med = 0;
+    __syncthreads();
+    for (int off = 1; off < 256; off <<= 1) {
+        int v = t >= off ? hs[t - off] : 0;
+        __syncthreads();
+        hs[t] += v;
+        __syncthreads();
+    }
+    if (t < bins && hs[t] > N / 2 && (t == 0 || hs[t - 1] <= N / 2)) med = t;
+    __syncthreads();
+    int m = med;
+    float s = bins == 64 ? 1.4826f * (float)(4 * m + 3) / 255.0f : 1.4826f * ((float)m + 0.75f) / 255.0f;
+    float acc[27];
+#pragma unroll
+    for (int k = 0; k < 27; k++) acc[k] = 0.0f;
+    for (int i = blockIdx.x * 256 + t; i < N; i += gridDim.x * 256) {
+        float2 rz = res[i];
+        float r = rz.x;
+        float u = fabsf(r) / s;
+        float w = 0.0f;
+        if (robust == 0) {
+            float bb = 4.6851f;
+            if (u <= bb) {
+                float ub = u / bb, ub2 = ub * ub;
+                w = (1.0f - ub2) * rz.y;
+            }
+        } else {
+            float k = 1.345f;
+            float hw = (u <= k) ? 1.0f : k / u;
+            w = sqrtf(hw) * rz.y;
+        }
+        float wr = r * w;
+        wts[i] = w;
+        float4 ja = jac[2 * i], jb = jac[2 * i + 1];
+        float jw[6] = {ja.x * w, ja.y * w, ja.z * w, ja.w * w, jb.x * w, jb.y * w};
+        int k = 0;
+#pragma unroll
+        for (int r0 = 0; r0 < 6; r0++)
+#pragma unroll
+            for (int c0 = r0; c0 < 6; c0++) acc[k++] += jw[r0] * jw[c0];
+#pragma unroll
+        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += jw[r0] * wr;
+    }
+    reduce27(acc, part);
+    if (last_group(ticket)) {
+        combine27(part, gridDim.x, sys);
+        __syncthreads();
+        if (t == 0 && solve_here) solve_update(sys, trel, cfg, N, 1);
+        // keep the histogram of this iteration (inspection) and clear it for the next
+        madhist[256 + t] = madhist[t];
+        madhist[t] = 0u;
+        if (t == 0) reset_ticket(ticket);
+    }
+}
+
+int photo_groups(int N) {
+    int g = N / 256;
+    return g < 1 ? 1 : (g > 512 ? 512 : g);
+}
+
+void launch_photo_iteration(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s) {
+    int G = photo_groups(d.N);
+    k_photo_res<<<G, 256, 0, s>>>(d.pts, d.jac[layer], d.N, layer, d.cur.gray[layer], d.W >> layer, d.H >> layer,
+                                  d.cur.z, d.W, d.H, d.calib, d.trel, c.mad_bins, d.res, d.madhist);
+    k_photo_sys<<<G, 256, 0, s>>>(d.res, d.jac[layer], d.N, c.mad_bins, c.robust, d.madhist, d.wts, d.part, d.sys,
+                                  d.trel, d.tickets + TK_PHOTO, c, solve_here ? 1 : 0);
+}
+
+// ------------------------------------------------------------------ A18 + A19
+// convertZMapToXYZ6Kernel (rgb2gray.cu:595-624) + computeNormalsCudaKernel
+// (d2z.cu:503-537): positions and normals recomputed from the depth map.
+__global__ __launch_bounds__(256) void k_ref_cloud(const float* __restrict__ z, int W, int H, const float* __restrict__ c,
+                                                   float4* __restrict__ pos, float4* __restrict__ nrm) {
+    int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    size_t o = (size_t)x + (size_t)y * W;
+    f3 p;
+    bool valid = backproject(z[o], x, y, c, p);
+    float mask = valid ? 1.0f : 0.0f;
+    float4 n = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (x < 1 || y < 1 || x > W - 2 || y > H - 2) {
+        mask = 0.0f;
+    } else {
+        f3 a, b2, cc, dd;
+        backproject(z[o - 1], x - 1, y, c, a);
+        backproject(z[o + 1], x + 1, y, c, b2);
+        backproject(z[o - W], x, y - 1, c, cc);
+        backproject(z[o + W], x, y + 1, c, dd);
+        if (a.z < 0.0f && b2.z < 0.0f && cc.z < 0.0f && dd.z < 0.0f) {
+            f3 u{b2.x - a.x, b2.y - a.y, b2.z - a.z};
+            f3 v{dd.x - cc.x, dd.y - cc.y, dd.z - cc.z};
+            float nx = u.y * v.z - u.z * v.y;
+            float ny = -(u.x * v.z - u.z * v.x);
+            float nz = u.x * v.y - u.y * v.x;
+            float len = nx * nx + ny * ny + nz * nz;
+            len = 1.0f / sqrtf(len);
+            n = make_float4(nx * len, ny * len, nz * len, 0.0f);
+        } else {
+            mask = 0.0f;
+        }
+    }
+    pos[o] = make_float4(p.x, p.y, p.z, mask);
+    nrm[o] = n;
+}
+
+// genPointsCloud6Kernel (d2z.cu:436-469): every z < 0 is valid.
+__global__ __launch_bounds__(256) void k_cur_cloud(const float* __restrict__ z, int W, int H, const float* __restrict__ c,
+                                                   float4* __restrict__ pos) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= W * H) return;
+    int x = i % W, y = i / W;
+    float zz = z[i];
+    if (zz < 0.0f) {
+        float fx = c[KL], fy = c[KL + 4], cx = c[KL + 2], cy = c[KL + 5];
+        pos[i] = make_float4(((float)x - cx) * zz / fx, ((float)y - cy) * zz / fy, zz, 1.0f);
+    } else {
+        pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// downSampleCloud6Kernel (d2z.cu:557-588): masked 2x2 mean, normal renormalised.
+__global__ __launch_bounds__(256) void k_cloud_down(const float4* __restrict__ sp, const float4* __restrict__ sn, int Wd,
+                                                    int Hd, float4* __restrict__ dp, float4* __restrict__ dn) {
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Wd * Hd) return;
+    int x = i % Wd, y = i / Wd;
+    int Ws = Wd * 2;
+    size_t h = (size_t)x * 2 + (size_t)y * 2 * Ws;
+    size_t hs[4] = {h, h + 1, h + Ws, h + Ws + 1};
+    float d[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float n = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        float4 p = sp[hs[t]];
+        if (p.w != 0.0f) {
+            n++;
+            d[0] += p.x; d[1] += p.y; d[2] += p.z;
+            if (sn) { float4 q = sn[hs[t]]; d[3] += q.x; d[4] += q.y; d[5] += q.z; }
+        }
+    }
+    if (n > 0.0f) {
+        for (int k = 0; k < 6; k++) d[k] /= n;
+        dp[i] = make_float4(d[0], d[1], d[2], 1.0f);
+        if (dn) {
+            float len = d[3] * d[3] + d[4] * d[4] + d[5] * d[5];
+            len = 1.0f / sqrtf(len);
+            dn[i] = make_float4(d[3] * len, d[4] * len, d[5] * len, 0.0f);
+        }
+    } else {
+        dp[i] = make_float4(d[0], d[1], d[2], 0.0f);
+        if (dn) dn[i] = make_float4(d[3], d[4], d[5], 0.0f);
+    }
+}
+
+void launch_ref_cloud(const Dev& d, hipStream_t s) {
+    dim3 g16((d.W + 15) / 16, (d.H + 15) / 16);
+    k_ref_cloud<<<g16, 256, 0, s>>>(d.ref.z, d.W, d.H, d.calib, d.rpos[0], d.rnrm[0]);
+}
+void launch_ref_cloud_pyramid(const Dev& d, hipStream_t s) {
+    for (int l = 1; l < d.L; l++) {
+        int n = (d.W >> l) * (d.H >> l);
+        k_cloud_down<<<(n + 255) / 256, 256, 0, s>>>(d.rpos[l - 1], d.rnrm[l - 1], d.W >> l, d.H >> l, d.rpos[l],
+                                                     d.rnrm[l]);
+    }
+}
+void launch_cur_cloud(const Dev& d, hipStream_t s) {
+    int n = d.W * d.H;
+    k_cur_cloud<<<(n + 255) / 256, 256, 0, s>>>(d.cur.z, d.W, d.H, d.calib, d.cpos[0]);
+    for (int l = 1; l < d.L; l++) {
+        int m = (d.W >> l) * (d.H >> l);
+        k_cloud_down<<<(m + 255) / 256, 256, 0, s>>>(d.cpos[l - 1], nullptr, d.W >> l, d.H >> l, d.cpos[l], nullptr);
+    }
+}
+
+// ------------------------------------------------------------------ A20 + A21
+// icpResidualKernel2 (rgb2gray.cu:223-260) over 1024-pixel row-major chunks;
+// the valid flags are kept as wave ballots for the ordered compaction.
+__global__ __launch_bounds__(1024) void k_icp_mask(const float4* __restrict__ rp, const float4* __restrict__ rn, int w,
+                                                   int h, int layer, const float* __restrict__ trel,
+                                                   const float* __restrict__ c, const float4* __restrict__ cp,
+                                                   float* __restrict__ rfull, uint64_t* __restrict__ bits,
+                                                   int* __restrict__ cnt) {
+    __shared__ int wc[16];
+    float T[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) T[k] = trel[k];
+    int o = blockIdx.x * 1024 + threadIdx.x;
+    bool valid = false;
+    if (o < w * h) {
+        float4 p = rp[o];
+        if (p.w != 0.0f) {
+            float4 nn = rn[o];
+            f3 p3{p.x, p.y, p.z}, n3{nn.x, nn.y, nn.z};
+            f3 q3 = mul4(T, p3);
+            f3 cn = rot4(T, n3);
+            const float* K = c + KL;
+            float a = 1.0f / (float)(1 << layer), b = 0.5f * (a - 1.0f);
+            float qx = q3.x / q3.z, qy = q3.y / q3.z;
+            float px = a * (K[0] * qx + K[2]) + b;
+            float py = a * (K[4] * qy + K[5]) + b;
+            if (px >= 0.0f && px < (float)(w - 1) && py >= 0.0f && py < (float)(h - 1)) {
+                int xdi = (int)px, ydi = (int)py;
+                int o2 = xdi + ydi * w;
+                float4 c0 = cp[o2], c1 = cp[o2 + 1], c2 = cp[o2 + w], c3 = cp[o2 + w + 1];
+                if (c0.w != 0.0f && c1.w != 0.0f && c2.w != 0.0f && c3.w != 0.0f) {
+                    float fx = px - (float)xdi, fy = py - (float)ydi;
+                    f3 cc{blend(c0.x, c1.x, c2.x, c3.x, fx, fy), blend(c0.y, c1.y, c2.y, c3.y, fx, fy),
+                          blend(c0.z, c1.z, c2.z, c3.z, fx, fy)};
+                    f3 dd{cc.x - q3.x, cc.y - q3.y, cc.z - q3.z};
+                    rfull[o] = dd.x * cn.x + dd.y * cn.y + dd.z * cn.z;
+                    valid = true;
+                }
+            }
+        }
+    }
+    uint64_t bl = __ballot(valid);
+    int wave = threadIdx.x >> 6;
+    if (lane_id() == 0) {
+        bits[(size_t)blockIdx.x * 16 + wave] = bl;
+        wc[wave] = __popcll(bl);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int s = 0;
+        for (int q = 0; q < 16; q++) s += wc[q];
+        cnt[blockIdx.x] = s;
+    }
+}
+
+// selectValidIndex + packIndex (rgb2gray.cu:827-850): ascending compaction and
+// the squared residuals of the kept pixels.
+__global__ __launch_bounds__(1024) void k_icp_compact(const float* __restrict__ rfull, const uint64_t* __restrict__ bits,
+                                                      const int* __restrict__ cnt, int groups, int* __restrict__ sel,
+                                                      float* __restrict__ r2, int* __restrict__ nout) {
+    __shared__ int red[16];
+    __shared__ int wpre[16];
+    int g = blockIdx.x, t = threadIdx.x, wave = t >> 6, lane = lane_id();
+    int v = 0;
+    for (int q = t; q < g; q += 1024) v += cnt[q];
+    v = (int)wave_sum((float)v);
+    if (lane == 0) red[wave] = v;
+    uint64_t word = bits[(size_t)g * 16 + wave];
+    if (lane == 0) wpre[wave] = __popcll(word);
+    __syncthreads();
+    int prefix = 0;
+    for (int q = 0; q < 16; q++) prefix += red[q];
+    int wp = 0;
+    for (int q = 0; q < wave; q++) wp += wpre[q];
+    uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    if ((word >> lane) & 1ull) {
+        int k = prefix + wp + __popcll(word & lt);
+        int o = g * 1024 + t;
+        float r = rfull[o];
+        sel[k] = o;
+        r2[k] = r * r;
+    }
+    if (g == groups - 1 && t == 0) {
+        int tot = prefix;
+        for (int q = 0; q < 16; q++) tot += wpre[q];
+        *nout = tot;
+    }
+}
+
+// generateStudentTWeights (rgb2gray.cu:470-568, 854-877), round k in 1..3: the
+// weights of the previous round are recomputed from its variance, the products
+// are truncated to int per element (one element per thread) and summed by the
+// reference's 1024-wide tree; the last block combines the block sums.
+__global__ __launch_bounds__(1024) void k_student_round(const float* __restrict__ r2, const int* __restrict__ nptr,
+                                                        int round, float* var, float* bsum, unsigned* ticket) {
+    __shared__ float p[1024];
+    int n = *nptr;
+    int t = threadIdx.x;
+    int nb = (n + 1023) / 1024;
+    int i = blockIdx.x * 1024 + t;
+    float prev = round > 1 ? var[round - 2] : 0.0f;
+    int sum = 0;
+    if (i < n) {
+        float x = r2[i];
+        float w = round == 1 ? 6 / (5 + x / 100.0f) : 6 / (5 + x / (prev / (float)n));
+        sum += x * w;
+    }
+    p[t] = (float)sum;
+    __syncthreads();
+    for (int s = 512; s >= 1; s >>= 1) {
+        if (t < s) p[t] += p[t + s];
+        __syncthreads();
+    }
+    if (t == 0) bsum[blockIdx.x] = p[0];
+    if (last_group(ticket)) {
+        if (nb >= 1) {
+            if (nb == 1) {
+                if (t == 0) var[round - 1] = bsum[0];
+            } else {
+                int ss = 0;
+                for (int q = t; q < nb; q += 1024) ss += bsum[q];
+                p[t] = (float)ss;
+                __syncthreads();
+                for (int s = 512; s >= 1; s >>= 1) {
+                    if (t < s) p[t] += p[t + s];
+                    __syncthreads();
+                }
+                if (t == 0) {
+                    float v = p[0];
+                    float mn = (float)n;
+                    if (v < mn) v = mn;
+                    var[round - 1] = v;
+                }
+            }
+        }
+        if (t == 0) reset_ticket(ticket);
+    }
+}
+
+// icpResidualIndexKernel2 (rgb2gray.cu:323-390) with the final clamped
+// Student-t weight, and the 27 dot products of its system.
+__global__ __launch_bounds__(256) void k_icp_sys(const float4* __restrict__ rp, const float4* __restrict__ rn,
+                                                 const int* __restrict__ sel, const float* __restrict__ r2,
+                                                 const int* __restrict__ nptr, const float* __restrict__ var, int w,
+                                                 int h, int layer, const float* __restrict__ c,
+                                                 const float4* __restrict__ cp, float* part, double* sys, float* trel,
+                                                 unsigned* ticket, SolveCfg cfg, int solve_here, int N1) {
+    int n = *nptr;
+    float T[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) T[k] = trel[k];
+    float acc[27];
+#pragma unroll
+    for (int k = 0; k < 27; k++) acc[k] = 0.0f;
+    float v3 = var[2];
+    const float* K = c + KL;
+    float a = 1.0f / (float)(1 << layer), b = 0.5f * (a - 1.0f);
+    float s = cfg.scale_in;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        int o = sel[i];
+        float4 p = rp[o], nn = rn[o];
+        f3 p3{p.x, p.y, p.z}, rnv{nn.x, nn.y, nn.z};
+        f3 q3 = mul4(T, p3);
+        f3 cn = rot4(T, rnv);
+        float qx = q3.x / q3.z, qy = q3.y / q3.z;
+        float px = a * (K[0] * qx + K[2]) + b;
+        float py = a * (K[4] * qy + K[5]) + b;
+        float res = 0.0f, J[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+        if (px >= 0.0f && px < (float)(w - 1) && py >= 0.0f && py < (float)(h - 1)) {
+            float x2 = r2[i];
+            float wt = fminf(6 / (5 + x2 / (v3 / (float)n)), 1.0f);
+            int xdi = (int)px, ydi = (int)py;
+            float fx = px - (float)xdi, fy = py - (float)ydi;
+            int o2 = xdi + ydi * w;
+            float4 c0 = cp[o2], c1 = cp[o2 + 1], c2 = cp[o2 + w], c3 = cp[o2 + w + 1];
+            f3 cc{blend(c0.x, c1.x, c2.x, c3.x, fx, fy), blend(c0.y, c1.y, c2.y, c3.y, fx, fy),
+                  blend(c0.z, c1.z, c2.z, c3.z, fx, fy)};
+            if (c0.w != 0.0f && wt > 0.0f && fabsf(cc.z - q3.z) < 300.0f) {
+                f3 d{cc.x - q3.x, cc.y - q3.y, cc.z - q3.z};
+                res = s * wt * (d.x * cn.x + d.y * cn.y + d.z * cn.z);
+                f3 gp[3] = {{0.0f, -p3.z, p3.y}, {p3.z, 0.0f, -p3.x}, {-p3.y, p3.x, 0.0f}};
+                f3 gn[3] = {{0.0f, -rnv.z, rnv.y}, {rnv.z, 0.0f, -rnv.x}, {-rnv.y, rnv.x, 0.0f}};
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    f3 dB = rot4(T, gp[k]);
+                    f3 dN = rot4(T, gn[k]);
+                    J[k] = -s * wt * (-cn.x * dB.x - cn.y * dB.y - cn.z * dB.z + d.x * dN.x + d.y * dN.y + d.z * dN.z);
+                }
+                f3 e[3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    f3 dB = rot4(T, e[k]);
+                    J[3 + k] = -wt * (-cn.x * dB.x - cn.y * dB.y - cn.z * dB.z);
+                }
+            }
+        }
+        int k = 0;
+#pragma unroll
+        for (int r0 = 0; r0 < 6; r0++)
+#pragma unroll
+            for (int c0 = r0; c0 < 6; c0++) acc[k++] += J[r0] * J[c0];
+#pragma unroll
+        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += J[r0] * res;
+    }
+    reduce27(acc, part);
+    if (last_group(ticket)) {
+        combine27(part, gridDim.x, sys + 27);
+        __syncthreads();
+        if (threadIdx.x == 0 && solve_here && n > 0) solve_update(sys, trel, cfg, N1, n);
+        if (threadIdx.x == 0) reset_ticket(ticket);
+    }
+}
+
+int icp_groups(int npix) {
+    int g = (npix + 255) / 256;
+    return g < 1 ? 1 : (g > 1024 ? 1024 : g);
+}
+
+void launch_icp_select(const Dev& d, int layer, hipStream_t s) {
+    int w = d.W >> layer, h = d.H >> layer, npix = w * h;
+    int G = (npix + 1023) / 1024;
+    uint64_t* words = d.icp_bits;
+    k_icp_mask<<<G, 1024, 0, s>>>(d.rpos[layer], d.rnrm[layer], w, h, layer, d.trel, d.calib, d.cpos[layer], d.rfull,
+                                  words, d.icp_cnt);
+    k_icp_compact<<<G, 1024, 0, s>>>(d.rfull, words, d.icp_cnt, G, d.icp_sel[layer], d.icp_r2[layer],
+                                     d.icp_n + layer);
+    for (int r = 1; r <= 3; r++)
+        k_student_round<<<G, 1024, 0, s>>>(d.icp_r2[layer], d.icp_n + layer, r, d.icp_var + layer * 4, d.bsum,
+                                           d.tickets + TK_ST);
+}
+
+void launch_icp_system(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s) {
+    int w = d.W >> layer, h = d.H >> layer;
+    int G = icp_groups(w * h);
+    k_icp_sys<<<G, 256, 0, s>>>(d.rpos[layer], d.rnrm[layer], d.icp_sel[layer], d.icp_r2[layer], d.icp_n + layer,
+                                d.icp_var + layer * 4, w, h, layer, d.calib, d.cpos[layer], d.part, d.sys, d.trel,
+                                d.tickets + TK_ICP, c, solve_here ? 1 : 0, d.N);
+}
+
+// ------------------------------------------------------------------ A25
+// Keyframe fusion: running weighted mean of the tracked frame's depth and gray
+// into the keyframe through the relative pose (30 mm gate, weight cap 16).
+__global__ __launch_bounds__(256) void k_fuse(float* __restrict__ kz, float* __restrict__ kg, float* __restrict__ kw,
+                                              const float* __restrict__ fz, const float* __restrict__ fg, int W, int H,
+                                              const float* __restrict__ trel, const float* __restrict__ c) {
+    int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= W || y >= H) return;
+    float T[16], iT[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) T[k] = trel[k];
+    iT[0] = T[0]; iT[1] = T[4]; iT[2] = T[8];   iT[3] = -(T[0] * T[3] + T[4] * T[7] + T[8] * T[11]);
+    iT[4] = T[1]; iT[5] = T[5]; iT[6] = T[9];   iT[7] = -(T[1] * T[3] + T[5] * T[7] + T[9] * T[11]);
+    iT[8] = T[2]; iT[9] = T[6]; iT[10] = T[10]; iT[11] = -(T[2] * T[3] + T[6] * T[7] + T[10] * T[11]);
+    iT[12] = 0.0f; iT[13] = 0.0f; iT[14] = 0.0f; iT[15] = 1.0f;
+    size_t o = (size_t)x + (size_t)y * W;
+    f3 p;
+    if (!backproject(kz[o], x, y, c, p)) return;
+    f3 q = mul4(T, p);
+    const float* KLm = c + KL;
+    float u = KLm[0] * (q.x / q.z) + KLm[2];
+    float v = KLm[4] * (q.y / q.z) + KLm[5];
+    if (!(u >= 0.0f && v >= 0.0f && u < (float)(W - 1) && v < (float)(H - 1))) return;
+    int xi = (int)(u + 0.5f), yi = (int)(v + 0.5f);
+    size_t fo = (size_t)xi + (size_t)yi * W;
+    float zf = fz[fo];
+    if (!(zf < -c[MIND] && zf > -c[MAXD])) return;
+    float dz = zf - q.z;
+    if (!(fabsf(dz) < 30.0f)) return;
+    f3 f4;
+    backproject(zf, xi, yi, c, f4);
+    f3 pk = mul4(iT, f4);
+    int xd = (int)u, yd = (int)v;
+    float gobs = bilerp(fg, W, xd, yd, u - (float)xd, v - (float)yd);
+    float Wold = kw[o];
+    float Wn = Wold + 1.0f;
+    kz[o] = (Wold * kz[o] + pk.z) / Wn;
+    kg[o] = (Wold * kg[o] + gobs) / Wn;
+    kw[o] = fminf(Wn, 16.0f);
+}
+void launch_fuse(const Dev& d, hipStream_t s) {
+    dim3 g16((d.W + 15) / 16, (d.H + 15) / 16);
+    k_fuse<<<g16, 256, 0, s>>>(d.ref.z, d.ref.gray[0], d.kf_w, d.cur.z, d.cur.gray[0], d.W, d.H, d.trel, d.calib);
+}
+
+// ------------------------------------------------------------------ A17
+// PhotoTracker::applyIncrement (phototracker.cpp:514-531) on the device for
+// sequences: pose[0..16) holds the host relative pose T (as float values),
+// pose[16..32) the global pose.
+__global__ void k_apply_increment(const float* trel, double* pose, double* out) {
+    if (threadIdx.x != 0) return;
+    double Tp[16], iTp[16], iT[16], Td[16], Tinc[16], G[16];
+    for (int i = 0; i < 16; i++) { Tp[i] = pose[i]; iT[i] = trel[i]; G[i] = pose[16 + i]; }
+    auto inv = [](const double* M, double* Mi) {
+        Mi[0] = M[0]; Mi[1] = M[4]; Mi[2] = M[8];   Mi[3] = -(M[0] * M[3] + M[4] * M[7] + M[8] * M[11]);
+        Mi[4] = M[1]; Mi[5] = M[5]; Mi[6] = M[9];   Mi[7] = -(M[1] * M[3] + M[5] * M[7] + M[9] * M[11]);
+        Mi[8] = M[2]; Mi[9] = M[6]; Mi[10] = M[10]; Mi[11] = -(M[2] * M[3] + M[6] * M[7] + M[10] * M[11]);
+        Mi[12] = 0; Mi[13] = 0; Mi[14] = 0; Mi[15] = 1;
+    };
+    auto mul = [](const double* A, const double* B, double* C) {
+        double t[16];
+        for (int i = 0; i < 4; i++)
+            for (int j = 0; j < 4; j++)
+                t[i * 4 + j] = A[i * 4 + 0] * B[0 * 4 + j] + A[i * 4 + 1] * B[1 * 4 + j] + A[i * 4 + 2] * B[2 * 4 + j] +
+                               A[i * 4 + 3] * B[3 * 4 + j];
+        for (int i = 0; i < 16; i++) C[i] = t[i];
+    };
+    inv(Tp, iTp);
+    inv(iT, Td);
+    mul(iTp, Td, Tinc);
+    mul(G, Tinc, G);
+    for (int i = 0; i < 16; i++) {
+        pose[i] = (double)(float)Td[i];
+        pose[16 + i] = G[i];
+        if (out) out[i] = G[i];
+    }
+}
+void launch_apply_increment(const Dev& d, double* pose_out, hipStream_t s) {
+    k_apply_increment<<<1, 64, 0, s>>>(d.trel, d.pose, pose_out);
+}
+
+}  // namespace rt

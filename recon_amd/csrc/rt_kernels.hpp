@@ -1,0 +1,76 @@
+// Host-side launchers of the tracker kernels (rt_kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rt {
+
+// Device buffers of one image pyramid.
+struct Pyr {
+    float* gray[4];
+    float* z;        // level-0 depth (z < 0 valid)
+};
+
+// Device state the kernels of one tracker share.
+struct Dev {
+    int W, H, L, N;             // level-0 size, levels, photometric points
+    const float* calib;         // 96 floats
+    Pyr ref, cur;
+    float* kf_w;                // fusion weights W*H
+    // selection
+    float* gmag;                // W*H
+    unsigned* ghist;            // 256
+    int* seg;                   // 512 cntA | 512 cntE | 512 startA | 512 startE | 512 mcTrim | 4 (thr, excess, totalA, total)
+    int* sel;                   // N
+    float4* pts;                // N
+    float4* jac[4];             // 2N per level
+    // photometric iteration
+    float2* res;                // N: r, zWeight
+    float* wts;                 // N: robust weight of the last iteration
+    unsigned* madhist;          // 512: accumulating histogram | copy of the last one
+    float* part;                // partial sums: max groups * 27
+    double* sys;                // 54: photometric 27 | geometric 27
+    float* trel;                // 16
+    unsigned* tickets;          // 16 counters
+    // geometric
+    float4* rpos[4];            // ref cloud (x,y,z,mask)
+    float4* rnrm[4];            // ref normals
+    float4* cpos[4];            // current cloud (x,y,z,mask)
+    float* rfull;               // W*H: residual of the full-image pass
+    int* icp_cnt;               // per-workgroup valid counts (W*H/1024)
+    uint64_t* icp_bits;         // valid-pixel ballots, 16 words per workgroup
+    int* icp_sel[4];            // per level W_l*H_l
+    float* icp_r2[4];           // per level
+    float* icp_var;             // 4 levels x 4 (3 rounds + spare)
+    int* icp_n;                 // 4
+    float* bsum;                // Student-t block sums (W*H/1024)
+    double* pose;               // device-side pose state for sequences: Tprev(16 float as double) | global(16)
+};
+
+struct SolveCfg {
+    int solver;       // 0 Cholesky, 1 CG
+    int mode;         // tracking mode
+    float lambda;
+    float scale_out;  // optScaleOut
+    int robust;
+    int mad_bins;
+    float scale_in;   // optScaleIn
+};
+
+void launch_convert_pyramid(const Dev& d, const Pyr& p, const uint8_t* g8, const uint16_t* d16, const float* rgbd,
+                            bool zero_madhist, bool zero_ghist, hipStream_t s);
+void launch_reset_pose(float* T, double* pose, hipStream_t s);
+void launch_fill(float* p, float v, int n, hipStream_t s);
+void launch_prepare_photo_reference(const Dev& d, hipStream_t s);
+void launch_ref_cloud(const Dev& d, hipStream_t s);
+void launch_ref_cloud_pyramid(const Dev& d, hipStream_t s);
+void launch_cur_cloud(const Dev& d, hipStream_t s);
+void launch_photo_iteration(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s);
+void launch_icp_select(const Dev& d, int layer, hipStream_t s);
+void launch_icp_system(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s);
+void launch_fuse(const Dev& d, hipStream_t s);
+void launch_apply_increment(const Dev& d, double* pose_out, hipStream_t s);
+int photo_groups(int N);
+int icp_groups(int npix);
+
+}  // namespace rt

@@ -1,0 +1,110 @@
+"""Seeded synthetic RGB-D inputs (see csrc/synth.cpp) as numpy arrays.
+
+Camera poses T_wc map camera points to world points in the tracker's
+convention (x right, y up, looking down -z, millimetres).
+"""
+import ctypes
+
+import numpy as np
+
+from . import synth_lib
+
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def pinhole(width, height):
+    """Intrinsics (fx, fy, cx, cy) of the workloads: fx = fy = 525 at 640 wide,
+    scaled with the width (262.5 at 320), principal point at the centre."""
+    f = 525.0 * width / 640.0
+    return f, f, (width - 1) / 2.0, (height - 1) / 2.0
+
+
+def calib_block(width, height, fx=None, fy=None, cx=None, cy=None, zmin=300.0, zmax=7000.0):
+    """The 96-float calibration block for a pinhole camera (rt_calib_pinhole)."""
+    from . import lib
+    if fx is None:
+        fx, fy, cx, cy = pinhole(width, height)
+    out = (ctypes.c_float * 96)()
+    rc = lib().rt_calib_pinhole(fx, fy, cx, cy, zmin, zmax, out)
+    if rc != 0:
+        raise ValueError("bad intrinsics")
+    return np.frombuffer(out, dtype=np.float32).copy()
+
+
+def calib_block_np(width, height, fx=None, fy=None, cx=None, cy=None, zmin=300.0, zmax=7000.0):
+    """Same block built in numpy (no native library needed)."""
+    if fx is None:
+        fx, fy, cx, cy = pinhole(width, height)
+    c = np.zeros(96, np.float32)
+    K = np.array([-fx, 0, cx, 0, fy, cy, 0, 0, 1], np.float32)
+    iK = np.array([np.float32(1) / K[0], 0, -K[2] / K[0], 0, np.float32(1) / K[4], -K[5] / K[4], 0, 0, 1], np.float32)
+    c[0:9] = K; c[9:18] = iK; c[43:52] = K; c[52:61] = iK
+    c[26:42] = np.eye(4, dtype=np.float32).ravel(); c[64:80] = np.eye(4, dtype=np.float32).ravel()
+    c[62] = zmin; c[63] = zmax
+    return c
+
+
+class Scene:
+    def __init__(self, seed):
+        self._lib = synth_lib()
+        self._h = self._lib.rs_scene_create(ctypes.c_uint64(seed))
+
+    def __del__(self):
+        try:
+            if self._h:
+                self._lib.rs_scene_destroy(self._h)
+        except Exception:
+            pass
+
+    def render(self, T_wc, width, height, noise_seed=0, noise=0.01, intrinsics=None):
+        fx, fy, cx, cy = intrinsics if intrinsics else pinhole(width, height)
+        T = np.ascontiguousarray(T_wc, dtype=np.float64).ravel()
+        gray = np.empty((height, width), np.uint8)
+        depth = np.empty((height, width), np.uint16)
+        self._lib.rs_render(self._h, T.ctypes.data_as(_dp), width, height, fx, fy, cx, cy,
+                            ctypes.c_uint64(noise_seed), noise, gray.ctypes.data, depth.ctypes.data)
+        return gray, depth
+
+
+def sample_motion(seed, theta_max_deg, t_max_mm):
+    M = np.zeros(16, np.float64)
+    synth_lib().rs_sample_motion(ctypes.c_uint64(seed), theta_max_deg, t_max_mm, M.ctypes.data_as(_dp))
+    return M.reshape(4, 4)
+
+
+def trajectory(seed, n, rot_max_deg, trans_max_mm, seg_len=0, slow_frac=1.0):
+    P = np.zeros((n, 16), np.float64)
+    synth_lib().rs_trajectory(ctypes.c_uint64(seed), n, rot_max_deg, trans_max_mm, seg_len, slow_frac,
+                              P.ctypes.data_as(_dp))
+    return P.reshape(n, 4, 4)
+
+
+def make_pair(seed, width, height, theta_max_deg, t_max_mm, noise=0.01):
+    """One frame pair: (ref gray, ref depth, cur gray, cur depth, Trel_gt) where
+    Trel_gt maps reference-camera points into the current camera."""
+    scene = Scene(seed)
+    M = sample_motion(seed * 7 + 1, theta_max_deg, t_max_mm)
+    T0 = np.eye(4)
+    T1 = T0 @ M
+    g0, d0 = scene.render(T0, width, height, noise_seed=seed * 2 + 1, noise=noise)
+    g1, d1 = scene.render(T1, width, height, noise_seed=seed * 2 + 2, noise=noise)
+    return g0, d0, g1, d1, np.linalg.inv(M)
+
+
+def make_sequence(seed, n, width, height, rot_max_deg, trans_max_mm, noise=0.01, seg_len=0, slow_frac=1.0):
+    """n frames along a smooth trajectory: gray (n,H,W) u8, depth (n,H,W) u16,
+    poses (n,4,4) T_wc with poses[0] = identity."""
+    scene = Scene(seed)
+    poses = trajectory(seed * 13 + 5, n, rot_max_deg, trans_max_mm, seg_len, slow_frac)
+    gray = np.empty((n, height, width), np.uint8)
+    depth = np.empty((n, height, width), np.uint16)
+    for k in range(n):
+        gray[k], depth[k] = scene.render(poses[k], width, height, noise_seed=seed * 1000 + k, noise=noise)
+    return gray, depth, poses
+
+
+def pose_error(T_est, T_gt):
+    """(rotation error in degrees, translation error in mm) of two 4x4 poses."""
+    D = np.linalg.inv(np.asarray(T_gt, np.float64)) @ np.asarray(T_est, np.float64)
+    c = np.clip((np.trace(D[:3, :3]) - 1.0) / 2.0, -1.0, 1.0)
+    return float(np.degrees(np.arccos(c))), float(np.linalg.norm(D[:3, 3]))
