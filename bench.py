@@ -1,0 +1,213 @@
+#!/usr/bin/env python3
+"""Benchmark: tracked frames/s at 640x480, batched persistent tracker vs the
+plain single-pair tracker on the same synthetic pairs.
+
+A step is one pass of the hot path over one batch: keyframe preparation
+(conversion, pyramids, gradient histogram, point selection, Jacobians) and the
+coarse-to-fine photometric Gauss-Newton loop for every pair of the batch, from
+raw 8-bit gray / 16-bit depth frames resident in HBM to converged poses on the
+host. Prints one JSON line (see the repository's benchmark contract).
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
+BYTES_PER_POINT_ITER = 60  # per selected point per GN iteration (point 16 + J 24 + 4 taps 16 + depth tap 4)
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_inputs(npairs, W, H, theta, tmax, seed0):
+    """Render npairs (reference, current) frames on the GPU into two device
+    buffers each for gray and depth; returns buffers, ground-truth Trel and
+    intrinsics."""
+    from recon_amd import synth_lib
+    from recon_amd.hip import DeviceBuffer, hip
+    from recon_amd.synth import pinhole, sample_motion
+    S = synth_lib()
+    fx, fy, cx, cy = pinhole(W, H)
+    npx = W * H
+    g = DeviceBuffer(2 * npairs * npx)
+    d = DeviceBuffer(2 * npairs * npx * 2)
+    gts = []
+    dp = ctypes.POINTER(ctypes.c_double)
+    nscenes = min(npairs, 64)
+    scenes = [S.rs_scene_create(ctypes.c_uint64(seed0 + k)) for k in range(nscenes)]
+    for i in range(npairs):
+        sc = scenes[i % nscenes]
+        M = sample_motion(seed0 * 1000 + i, theta, tmax)
+        T0 = np.eye(4).ravel().astype(np.float64)
+        T1 = M.ravel().astype(np.float64)
+        for f, T in ((0, T0), (1, T1)):
+            k = 2 * i + f
+            rc = S.rs_render_device(sc, T.ctypes.data_as(dp), W, H, fx, fy, cx, cy, ctypes.c_uint64(seed0 * 7919 + k),
+                                    0.01, g.ptr + k * npx, d.ptr + k * npx * 2, None)
+            if rc != 0:
+                raise RuntimeError(f"render failed: {rc}")
+        gts.append(np.linalg.inv(M))
+    hip().hipDeviceSynchronize()
+    for sc in scenes:
+        S.rs_scene_destroy(sc)
+    return g, d, np.array(gts)
+
+
+def pose_errors(Trel_est, Trel_gt):
+    from recon_amd.synth import pose_error
+    e = np.array([pose_error(a, b) for a, b in zip(Trel_est, Trel_gt)])
+    return e[:, 0], e[:, 1]
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--pairs", type=int, default=4096)
+    ap.add_argument("--plain-pairs", type=int, default=64)
+    ap.add_argument("--width", type=int, default=640)
+    ap.add_argument("--height", type=int, default=480)
+    ap.add_argument("--theta", type=float, default=3.0, help="max rotation (deg) of a pair")
+    ap.add_argument("--tmax", type=float, default=50.0, help="max translation (mm) of a pair")
+    ap.add_argument("--cpu-pairs", type=int, default=8, help="pairs timed on the CPU baseline (0: skip)")
+    ap.add_argument("--seed", type=int, default=4)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(pairs are independent: shard them across GPUs and gather poses)", file=sys.stderr)
+        sys.exit(2)
+
+    from recon_amd import PHOTOMETRIC, RtPair
+    from recon_amd.hip import Event, stream_sync
+    from recon_amd.synth import calib_block
+    from recon_amd.tracker import BatchTracker, PhotoTracker
+
+    W, H, B = args.width, args.height, args.pairs
+    iters = (1, 3, 10, 5)
+    N = 8192
+    L = 4 if W == 640 else 3
+    t0 = time.time()
+    g, d, gt = make_inputs(B, W, H, args.theta, args.tmax, args.seed)
+    log(f"inputs: {B} pairs {W}x{H} rendered in {time.time() - t0:.1f}s")
+    npx = W * H
+    cal = calib_block(W, H)
+
+    # ---- plain single-pair tracker (same kernels, one pair at a time)
+    P = min(args.plain_pairs, B)
+    tr = PhotoTracker(W, H, cal, mode=PHOTOMETRIC, n_select=N, iters=iters)
+    plain_T = []
+
+    def plain_pair(i):
+        tr.set_reference_points(g.ptr + 2 * i * npx, d.ptr + 2 * i * npx * 2, mode=PHOTOMETRIC, force=True, on_device=True)
+        tr.set_current_image(g.ptr + (2 * i + 1) * npx, d.ptr + (2 * i + 1) * npx * 2, on_device=True)
+        tr.track()
+        return tr.trel().astype(np.float64)
+
+    for i in range(min(4, P)):
+        plain_pair(i)
+    tr.synchronize()
+    t1 = time.perf_counter()
+    for i in range(P):
+        plain_T.append(plain_pair(i))
+    tr.synchronize()
+    plain_s = time.perf_counter() - t1
+    plain_fps = P / plain_s
+    log(f"plain single-pair: {P} pairs in {plain_s * 1e3:.1f} ms -> {plain_fps:.1f} frames/s")
+
+    # ---- batched persistent tracker
+    bt = BatchTracker(B, W, H, mode=PHOTOMETRIC, n_select=N, iters=iters)
+    pairs = [dict(ref_gray=g.ptr + 2 * i * npx, ref_depth=d.ptr + 2 * i * npx * 2, cur_gray=g.ptr + (2 * i + 1) * npx,
+                  cur_depth=d.ptr + (2 * i + 1) * npx * 2, width=W, height=H, calib=cal) for i in range(B)]
+    prepared = bt.prepare(pairs)
+    for _ in range(max(1, args.warmup)):
+        Trel_b, _, rms, st = bt.track(prepared)
+    stream = bt.L.rt_batch_stream(bt.h)
+    stream_sync(stream)
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        Trel_b, _, rms, st = bt.track(prepared)
+    stream_sync(stream)
+    step_s = (time.perf_counter() - t2) / args.steps
+    value = B / step_s
+
+    # ---- roofline of the dominant kernel (the persistent tracking launch), HIP events on its stream
+    ev0, ev1 = Event(), Event()
+    reps = 3
+    ev0.record(stream)
+    for _ in range(reps):
+        bt.L.rt_batch_retrack(bt.h, None, None)
+    ev1.record(stream)
+    ev1.synchronize()
+    k_ms = ev0.elapsed_ms(ev1) / reps
+    alg_bytes = B * sum(iters[:L]) * N * BYTES_PER_POINT_ITER
+    achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+
+    # ---- errors
+    rb, tb = pose_errors(Trel_b, gt)
+    rp, tp = pose_errors(plain_T, gt[:P])
+    ra, ta = pose_errors(Trel_b[:P], plain_T)
+    log(f"batched: {B} pairs, {step_s * 1e3:.2f} ms/step -> {value:.0f} frames/s ({value / plain_fps:.1f}x plain); "
+        f"kernel {k_ms:.2f} ms, {achieved:.0f} GB/s algorithmic")
+    log(f"pose error vs truth: batched median {np.median(rb):.4f} deg {np.median(tb):.3f} mm, "
+        f"plain median {np.median(rp):.4f} deg {np.median(tp):.3f} mm; batched vs plain max {ra.max():.2e} deg {ta.max():.2e} mm")
+
+    # ---- CPU baseline: the oracle's FP32 transcription on the host, single thread
+    cpu = None
+    if args.cpu_pairs > 0:
+        import oracle
+        from recon_amd.hip import hip
+        gh = g.to_array(np.uint8, 2 * args.cpu_pairs * npx)
+        dh = d.to_array(np.uint16, 2 * args.cpu_pairs * npx)
+        t3 = time.perf_counter()
+        for i in range(args.cpu_pairs):
+            o = oracle.Tracker(W, H, cal, mode=oracle.PHOTOMETRIC, levels=L, iters=iters, nselect=N, dtype=np.float32)
+            o.set_reference(gh[2 * i * npx:(2 * i + 1) * npx].reshape(H, W), dh[2 * i * npx:(2 * i + 1) * npx].reshape(H, W))
+            o.set_current(gh[(2 * i + 1) * npx:(2 * i + 2) * npx].reshape(H, W), dh[(2 * i + 1) * npx:(2 * i + 2) * npx].reshape(H, W))
+            o.track()
+        cpu_s = time.perf_counter() - t3
+        cpu = {"value": round(args.cpu_pairs / cpu_s, 3), "unit": "frames/s", "cores": 1, "kind": "port",
+               "sample": f"{args.cpu_pairs} of the {W}x{H} pairs, FP32 oracle transcription, one thread ({cpu_s:.1f} s)"}
+        log(f"cpu baseline: {cpu['value']} frames/s")
+
+    out = {
+        "metric": "tracked frames/s at 640x480 (learned+batched vs plain GN) at matched pose error",
+        "value": round(value, 1),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(step_s * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {"workload": f"batched photometric tracking of {B} frame pairs at {W}x{H} (rotation<={args.theta} deg, "
+                               f"translation<={args.tmax} mm), {N} points, {L} levels, iterations {list(iters[:L])}, identity start",
+                   "pairs": B, "width": W, "height": H, "points": N, "levels": L},
+        "plain_fps": round(plain_fps, 1),
+        "speedup_vs_plain": round(value / plain_fps, 2),
+        "pose_error": {"batched_median_deg": float(np.median(rb)), "batched_median_mm": float(np.median(tb)),
+                       "plain_median_deg": float(np.median(rp)), "plain_median_mm": float(np.median(tp)),
+                       "batched_vs_plain_max_deg": float(ra.max()), "batched_vs_plain_max_mm": float(ta.max())},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "kb_track (persistent Gauss-Newton)", "kernel_ms": round(k_ms, 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

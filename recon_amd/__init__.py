@@ -10,7 +10,6 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librecon_amd.so")
-SYNTH_PATH = os.path.join(_HERE, "librecon_synth.so")
 
 _lib = None
 _synth = None
@@ -107,11 +106,10 @@ def lib():
 
 
 def synth_lib():
+    """The synthetic-scene entry points (rs_*) of the same library."""
     global _synth
     if _synth is None:
-        if not os.path.exists(SYNTH_PATH):
-            raise RuntimeError(f"{SYNTH_PATH} is missing: build it with `python -m recon_amd.build`")
-        S = ctypes.CDLL(SYNTH_PATH)
+        S = lib()
         S.rs_scene_create.restype = c_void_p
         S.rs_scene_create.argtypes = [ctypes.c_uint64]
         S.rs_scene_destroy.argtypes = [c_void_p]
@@ -119,6 +117,9 @@ def synth_lib():
                                 ctypes.c_uint64, c_double, c_void_p, c_void_p]
         S.rs_sample_motion.argtypes = [ctypes.c_uint64, c_double, c_double, P(c_double)]
         S.rs_trajectory.argtypes = [ctypes.c_uint64, c_int, c_double, c_double, c_int, c_double, P(c_double)]
+        S.rs_render_device.restype = c_int
+        S.rs_render_device.argtypes = [c_void_p, P(c_double), c_int, c_int, c_double, c_double, c_double, c_double,
+                                       ctypes.c_uint64, c_double, c_void_p, c_void_p, c_void_p]
         _synth = S
     return _synth
 

@@ -1,7 +1,7 @@
 """Build the tracker's native libraries in-tree.
 
-librecon_amd.so   HIP kernels + C ABI (hipcc, gfx950)
-librecon_synth.so seeded synthetic RGB-D generator (g++, OpenMP)
+librecon_amd.so   HIP kernels + C ABI (hipcc, gfx950), including the seeded
+                  synthetic RGB-D renderer (host and device)
 """
 import os
 import subprocess
@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["rt_kernels.hip", "rt_tracker.hip", "rt_batch.hip"]
+HIP_SOURCES = ["rt_kernels.hip", "rt_tracker.hip", "rt_batch.hip", "synth.hip"]
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
     # every FP32 value is the rounded result of the written operation sequence
@@ -38,14 +38,8 @@ def build_hip():
     return out
 
 
-def build_synth():
-    out = os.path.join(HERE, "librecon_synth.so")
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-o", out, os.path.join(CSRC, "synth.cpp")])
-    return out
-
-
 def build_all():
-    return [build_synth(), build_hip()]
+    return [build_hip()]
 
 
 if __name__ == "__main__":

@@ -1,0 +1,534 @@
+// Batched tracking of many independent frame pairs of mixed sizes.
+//
+// Keyframe preparation runs as batched versions of the single-pair kernels
+// (grid z or y = pair), sharing their device bodies (rt_prep.hpp), so every
+// per-pixel and per-point value is the same as on the single-pair path. The
+// Gauss-Newton loop runs in ONE persistent launch: each workgroup takes pairs
+// from a device work queue (ordered by decreasing cost) and runs the whole
+// coarse-to-fine photometric loop of a pair with workgroup barriers only:
+// residuals stay in registers between the residual and the system pass, the MAD
+// histogram lives in LDS, the 27 normal-equation terms are reduced in FP32 per
+// wave and combined across waves in FP64, and one lane runs the FP64 solve and
+// SE(3) update. Workgroup size and points per thread are template parameters
+// picked per batch by a timing autotuner; the pipeline is captured once into a
+// hipGraph and replayed.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <algorithm>
+#include <map>
+#include <string>
+#include <vector>
+#include "../../include/rt_tracker.h"
+#include "rt_kernels.hpp"
+#include "rt_prep.hpp"
+#include "rt_host.hpp"
+
+namespace rt {
+
+struct PairDesc {
+    int W, H, L, N;
+    int iters[4];
+    const float* calib;
+    const uint8_t* g8[2];
+    const uint16_t* d16[2];
+    float* gray[2][4];
+    float* z[2];
+    float* gmag;
+    unsigned* ghist;
+    int* seg;
+    int* sel;
+    float4* pts;
+    float4* jac[4];
+    const float* trel_in;   // 16
+    float* trel_out;        // 16
+    float* rms;             // 1
+    int* status;            // 1
+};
+
+// ------------------------------------------------------------------ batched preparation
+__global__ __launch_bounds__(256) void kb_convert(const PairDesc* __restrict__ P, int which) {
+    const PairDesc& d = P[blockIdx.z];
+    if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 64 >= d.H) return;
+    convert_pyramid_block<0>(blockIdx.x, blockIdx.y, d.g8[which], d.d16[which], nullptr, d.W, d.H, d.L, d.calib,
+                             d.gray[which][0], d.gray[which][1], d.gray[which][2], d.gray[which][3], d.z[which]);
+    if (which == 0 && blockIdx.x == 0 && blockIdx.y == 0) d.ghist[threadIdx.x] = 0u;
+}
+__global__ __launch_bounds__(256) void kb_features(const PairDesc* __restrict__ P) {
+    const PairDesc& d = P[blockIdx.z];
+    if ((int)blockIdx.x * 16 >= d.W || (int)blockIdx.y * 16 >= d.H) return;
+    ref_features_block(blockIdx.x, blockIdx.y, d.z[0], d.gray[0][0], d.W, d.H, d.calib, d.gmag, d.ghist);
+}
+__global__ __launch_bounds__(256) void kb_select_count(const PairDesc* __restrict__ P) {
+    const PairDesc& d = P[blockIdx.y];
+    select_count_block(blockIdx.x, d.gmag, d.W * d.H, d.ghist, d.N, d.seg);
+}
+__global__ __launch_bounds__(512) void kb_select_scan(const PairDesc* __restrict__ P) {
+    select_scan_block(P[blockIdx.x].seg);
+}
+__global__ __launch_bounds__(256) void kb_select_scatter(const PairDesc* __restrict__ P) {
+    const PairDesc& d = P[blockIdx.y];
+    select_scatter_block(blockIdx.x, d.gmag, d.W * d.H, d.N, d.seg, d.sel);
+}
+__global__ __launch_bounds__(256) void kb_ref_points(const PairDesc* __restrict__ P) {
+    const PairDesc& d = P[blockIdx.y];
+    int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= d.N) return;
+    ref_point(i, d.z[0], d.gray[0][0], d.gray[0][1], d.gray[0][2], d.gray[0][3], d.W, d.H, d.L, d.N, d.calib, 1e-3f,
+              d.sel, d.pts, d.jac[0], d.jac[1], d.jac[2], d.jac[3]);
+}
+__global__ void kb_reset_queue(int* q) {
+    if (threadIdx.x == 0) q[0] = 0;
+}
+
+// ------------------------------------------------------------------ persistent tracking
+struct TrackCfg {
+    int solver;
+    int robust;
+    int mad_bins;
+    float scale_out;
+};
+
+template <int BS, int PPT>
+__global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, const int* __restrict__ order,
+                                               int npairs, int* queue, TrackCfg cfg) {
+    constexpr int NW = BS / 64;
+    __shared__ float cal[96];
+    __shared__ float T[16];
+    __shared__ unsigned hist[256];
+    __shared__ double wsum[NW][29];
+    __shared__ int s_pair, s_med;
+    __shared__ int s_ok;
+    const int t = threadIdx.x, wave = t >> 6, lane = lane_id();
+    for (;;) {
+        __syncthreads();
+        if (t == 0) {
+            int q = atomicAdd(queue, 1);
+            s_pair = q < npairs ? order[q] : -1;
+        }
+        __syncthreads();
+        int p = s_pair;
+        if (p < 0) break;
+        const PairDesc& d = P[p];
+        const int W = d.W, H = d.H, L = d.L, N = d.N;
+        if (t < 96) cal[t] = d.calib[t];
+        if (t < 16) T[t] = d.trel_in[t];
+        if (t == 0) s_ok = 1;
+        __syncthreads();
+        float rms_last = 0.0f;
+        for (int layer = L - 1; layer >= 0; layer--) {
+            const int nit = d.iters[L - 1 - layer];
+            const float* img = d.gray[1][layer];
+            const float4* __restrict__ jac = d.jac[layer];
+            const int wl = W >> layer, hl = H >> layer;
+            const float a = 1.0f / (float)(1 << layer), b = 0.5f * (a - 1.0f);
+            for (int it = 0; it < nit; it++) {
+                for (int k = t; k < 256; k += BS) hist[k] = 0u;
+                float Tl[16];
+#pragma unroll
+                for (int k = 0; k < 16; k++) Tl[k] = T[k];
+                __syncthreads();
+                // residual pass: r and zWeight stay in registers
+                float r[PPT], zw[PPT];
+#pragma unroll
+                for (int k = 0; k < PPT; k++) {
+                    int i = t + k * BS;
+                    r[k] = 0.0f; zw[k] = 0.0f;
+                    if (i < N) {
+                        photo_point(d.pts[i], jac[2 * i + 1].z, Tl, a, b, img, wl, hl, d.z[1], W, H, cal, r[k], zw[k]);
+                        int bn = (int)((float)(cfg.mad_bins - 1) * fabsf(r[k]));
+                        if (bn >= 0 && bn < cfg.mad_bins) atomicAdd(&hist[bn], 1u);
+                    }
+                }
+                __syncthreads();
+                // median bin: first bin whose running count exceeds N/2 (histogram64.cu:48-61)
+                if (wave == 0) {
+                    unsigned c4[4];
+                    unsigned run = 0;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) { c4[q] = hist[lane * 4 + q]; run += c4[q]; }
+                    unsigned incl = run;
+#pragma unroll
+                    for (int o = 1; o < 64; o <<= 1) {
+                        unsigned v = __shfl_up(incl, o, 64);
+                        if (lane >= o) incl += v;
+                    }
+                    unsigned acc = incl - run;
+                    int found = 1 << 30;
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        acc += c4[q];
+                        if (found == (1 << 30) && (int)acc > N / 2) found = lane * 4 + q;
+                    }
+                    int m = found;
+#pragma unroll
+                    for (int o = 32; o >= 1; o >>= 1) m = min(m, __shfl_xor(m, o, 64));
+                    if (lane == 0) s_med = m == (1 << 30) ? 0 : m;
+                }
+                __syncthreads();
+                const float s = mad_scale(s_med, cfg.mad_bins);
+                float acc[29];
+#pragma unroll
+                for (int k = 0; k < 29; k++) acc[k] = 0.0f;
+#pragma unroll
+                for (int k = 0; k < PPT; k++) {
+                    int i = t + k * BS;
+                    if (i < N) {
+                        float w = robust_weight(r[k], zw[k], s, cfg.robust);
+                        float wr = r[k] * w;
+                        float4 ja = jac[2 * i], jb = jac[2 * i + 1];
+                        float jw[6] = {ja.x * w, ja.y * w, ja.z * w, ja.w * w, jb.x * w, jb.y * w};
+                        int q = 0;
+#pragma unroll
+                        for (int r0 = 0; r0 < 6; r0++)
+#pragma unroll
+                            for (int c0 = r0; c0 < 6; c0++) acc[q++] += jw[r0] * jw[c0];
+#pragma unroll
+                        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += jw[r0] * wr;
+                        acc[27] += wr * wr;
+                        acc[28] += w * w;
+                    }
+                }
+#pragma unroll
+                for (int k = 0; k < 29; k++) {
+                    float v = wave_sum(acc[k]);
+                    if (lane == 0) wsum[wave][k] = (double)v;
+                }
+                __syncthreads();
+                if (t == 0) {
+                    double sys[29];
+                    for (int k = 0; k < 29; k++) {
+                        double v = 0.0;
+                        for (int w2 = 0; w2 < NW; w2++) v += wsum[w2][k];
+                        sys[k] = v;
+                    }
+                    double A[36], bb[6], x[6];
+                    unpack27(sys, A, bb);
+                    bool ok = true;
+                    if (cfg.solver == 1) solve_cg(A, bb, x);
+                    else ok = solve_cholesky(A, bb, x);
+                    if (!ok) s_ok = 0;
+                    float TT[16];
+                    for (int k = 0; k < 16; k++) TT[k] = T[k];
+                    expm_update(x, TT, (double)cfg.scale_out);
+                    for (int k = 0; k < 16; k++) T[k] = TT[k];
+                    rms_last = sys[28] > 0.0 ? (float)sqrt(sys[27] / sys[28]) : 0.0f;
+                }
+                __syncthreads();
+            }
+        }
+        if (t < 16) d.trel_out[t] = T[t];
+        if (t == 0) { *d.rms = rms_last; *d.status = s_ok ? 0 : -1; }
+    }
+}
+
+typedef void (*TrackFn)(const PairDesc*, const int*, int, int*, TrackCfg);
+struct Variant { int bs, ppt; TrackFn fn; };
+static const Variant kVariants[] = {
+    {256, 8, kb_track<256, 8>},   {256, 16, kb_track<256, 16>}, {256, 32, kb_track<256, 32>},
+    {512, 4, kb_track<512, 4>},   {512, 8, kb_track<512, 8>},   {512, 16, kb_track<512, 16>},
+    {512, 32, kb_track<512, 32>}, {1024, 4, kb_track<1024, 4>}, {1024, 8, kb_track<1024, 8>},
+    {1024, 16, kb_track<1024, 16>}, {1024, 32, kb_track<1024, 32>},
+};
+
+}  // namespace rt
+
+using namespace rt;
+
+// ------------------------------------------------------------------ C ABI
+struct rt_batch {
+    int device = 0, max_pairs = 0, max_w = 0, max_h = 0;
+    rt_params prm;
+    hipStream_t stream = nullptr;
+    int cus = 256;
+    std::vector<void*> allocs;
+    // per-pair arenas
+    char* arena = nullptr;
+    size_t pair_bytes = 0;
+    PairDesc* desc_dev = nullptr;
+    int* order_dev = nullptr;
+    int* queue = nullptr;
+    float* calib_dev = nullptr;   // 96 per pair
+    float* trel_in = nullptr;     // 16 per pair
+    float* trel_out = nullptr;
+    float* rms = nullptr;
+    int* status = nullptr;
+    float* trel_in_host = nullptr;   // pinned staging
+    // last batch
+    int n = 0;
+    std::vector<PairDesc> desc;
+    int maxW = 0, maxH = 0, maxN = 0;
+    // tuning and graphs
+    std::map<std::string, std::pair<int, int>> tuned;  // key -> (variant, grid)
+    int variant = -1, grid = 0;
+    hipGraphExec_t graph_prep = nullptr, graph_track = nullptr;
+    int graph_n = -1, graph_variant = -1, graph_grid = -1;
+};
+
+static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, const rt_params* params,
+                               rt_batch** out) {
+    if (!out || max_pairs < 1 || max_w < 32 || max_h < 32) return set_error(RT_ERR_ARG, "rt_batch_create: bad argument");
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return set_error(RT_ERR_ARG, "no such device");
+    if (hipSetDevice(device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    rt_batch* b = new rt_batch();
+    b->device = device; b->max_pairs = max_pairs; b->max_w = max_w; b->max_h = max_h;
+    if (params) b->prm = *params; else rt_default_params(&b->prm);
+    if (b->prm.n_select < 256 || b->prm.n_select % 256 || b->prm.n_select > 32768) { delete b; return set_error(RT_ERR_ARG, "n_select must be a multiple of 256 up to 32768 for batches"); }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) == hipSuccess) b->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return set_error(RT_ERR_HIP, "stream"); }
+    size_t npx = (size_t)max_w * max_h;
+    size_t N = (size_t)b->prm.n_select;
+    (void)npx;
+    size_t per = 0;  // inputs are borrowed device pointers; the arena holds derived data only
+    per += 2 * align256(npx * 4 * 4 / 3 + 64);  // two gray pyramids
+    per += 2 * align256(npx * 4);               // two depth maps
+    per += align256(npx * 4);                   // gradient magnitudes
+    per += align256(256 * 4) + align256(2600 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(N * 32);
+    b->pair_bytes = per;
+    size_t total = per * (size_t)max_pairs;
+    void* q = nullptr;
+    if (hipMalloc(&q, total) != hipSuccess) { hipStreamDestroy(b->stream); delete b; return set_error(RT_ERR_NOMEM, "batch arena"); }
+    b->arena = (char*)q; b->allocs.push_back(q);
+    auto dal = [&](void** p, size_t bytes) {
+        if (hipMalloc(p, bytes) != hipSuccess) return false;
+        hipMemset(*p, 0, bytes);
+        b->allocs.push_back(*p);
+        return true;
+    };
+    bool ok = dal((void**)&b->desc_dev, sizeof(PairDesc) * max_pairs) && dal((void**)&b->order_dev, 4 * max_pairs) &&
+              dal((void**)&b->queue, 256) && dal((void**)&b->calib_dev, 96 * 4 * (size_t)max_pairs) &&
+              dal((void**)&b->trel_in, 64 * (size_t)max_pairs) && dal((void**)&b->trel_out, 64 * (size_t)max_pairs) &&
+              dal((void**)&b->rms, 4 * (size_t)max_pairs) && dal((void**)&b->status, 4 * (size_t)max_pairs);
+    if (!ok || hipHostMalloc((void**)&b->trel_in_host, 64 * (size_t)max_pairs, 0) != hipSuccess) {
+        for (void* p : b->allocs) hipFree(p);
+        hipStreamDestroy(b->stream);
+        delete b;
+        return set_error(RT_ERR_NOMEM, "batch buffers");
+    }
+    *out = b;
+    return RT_OK;
+}
+
+extern "C" void rt_batch_destroy(rt_batch* b) {
+    if (!b) return;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    if (b->graph_prep) hipGraphExecDestroy(b->graph_prep);
+    if (b->graph_track) hipGraphExecDestroy(b->graph_track);
+    for (void* p : b->allocs) hipFree(p);
+    if (b->trel_in_host) hipHostFree(b->trel_in_host);
+    hipStreamDestroy(b->stream);
+    delete b;
+}
+
+extern "C" void* rt_batch_stream(rt_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+static void launch_prep(rt_batch* b, hipStream_t s) {
+    int n = b->n;
+    dim3 g64((b->maxW + 63) / 64, (b->maxH + 63) / 64, n);
+    kb_convert<<<g64, 256, 0, s>>>(b->desc_dev, 0);
+    kb_convert<<<g64, 256, 0, s>>>(b->desc_dev, 1);
+    dim3 g16((b->maxW + 15) / 16, (b->maxH + 15) / 16, n);
+    kb_features<<<g16, 256, 0, s>>>(b->desc_dev);
+    kb_select_count<<<dim3(512, n), 256, 0, s>>>(b->desc_dev);
+    kb_select_scan<<<n, 512, 0, s>>>(b->desc_dev);
+    kb_select_scatter<<<dim3(512, n), 256, 0, s>>>(b->desc_dev);
+    kb_ref_points<<<dim3((b->maxN + 255) / 256, n), 256, 0, s>>>(b->desc_dev);
+}
+
+static TrackCfg track_cfg(const rt_batch* b) {
+    TrackCfg c;
+    c.solver = b->prm.solver;
+    c.robust = b->prm.robust;
+    c.mad_bins = b->prm.mad_bins;
+    c.scale_out = 1e3f;
+    return c;
+}
+
+static void launch_track(rt_batch* b, int variant, int grid, hipStream_t s) {
+    kb_reset_queue<<<1, 64, 0, s>>>(b->queue);
+    const Variant& v = kVariants[variant];
+    v.fn<<<grid, v.bs, 0, s>>>(b->desc_dev, b->order_dev, b->n, b->queue, track_cfg(b));
+}
+
+static std::string tune_key(const rt_batch* b) {
+    // batch size bucket and the size mix decide the best configuration
+    std::map<int, int> mix;
+    for (const PairDesc& d : b->desc) mix[d.W * 10000 + d.H]++;
+    char buf[64];
+    std::string k = std::to_string(b->n) + ":" + std::to_string(b->maxN);
+    for (auto& kv : mix) { snprintf(buf, sizeof(buf), "|%d*%d", kv.first, kv.second); k += buf; }
+    return k;
+}
+
+// Time every variant that covers the largest point count, at 1, 2 and 4
+// workgroups per CU, on the prepared batch; keep the fastest (per batch key).
+static int autotune(rt_batch* b) {
+    std::string key = tune_key(b);
+    auto it = b->tuned.find(key);
+    if (it != b->tuned.end()) { b->variant = it->second.first; b->grid = it->second.second; return RT_OK; }
+    const char* force = getenv("RT_BATCH_VARIANT");
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0); hipEventCreate(&e1);
+    float best = 1e30f;
+    int bv = -1, bg = 0;
+    for (int v = 0; v < (int)(sizeof(kVariants) / sizeof(kVariants[0])); v++) {
+        const Variant& var = kVariants[v];
+        if (var.bs * var.ppt < b->maxN) continue;
+        if (var.bs * var.ppt >= 4 * b->maxN && var.ppt > 4) continue;  // mostly idle lanes
+        if (force && atoi(force) != v) continue;
+        int per_cu_max = 2048 / var.bs;  // waves per CU stay within the hardware limit
+        for (int k = 1; k <= 4 && k <= per_cu_max; k *= 2) {
+            int grid = std::min(b->n, b->cus * k);
+            launch_track(b, v, grid, b->stream);  // warm
+            hipEventRecord(e0, b->stream);
+            launch_track(b, v, grid, b->stream);
+            hipEventRecord(e1, b->stream);
+            if (hipEventSynchronize(e1) != hipSuccess) { hipEventDestroy(e0); hipEventDestroy(e1); return set_error(RT_ERR_HIP, "autotune run failed"); }
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) { best = ms; bv = v; bg = grid; }
+            if (grid == b->n) break;
+        }
+    }
+    hipEventDestroy(e0); hipEventDestroy(e1);
+    if (bv < 0) return set_error(RT_ERR_UNSUPPORTED, "no batch kernel variant covers the point count");
+    b->tuned[key] = {bv, bg};
+    b->variant = bv; b->grid = bg;
+    if (getenv("RT_VERBOSE")) fprintf(stderr, "rt_batch autotune %s -> bs %d ppt %d grid %d (%.3f ms)\n", key.c_str(), kVariants[bv].bs, kVariants[bv].ppt, bg, best);
+    return RT_OK;
+}
+
+static int capture(rt_batch* b, hipGraphExec_t* out, bool prep) {
+    hipGraph_t g;
+    if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return set_error(RT_ERR_HIP, "begin capture");
+    if (prep) launch_prep(b, b->stream);
+    else launch_track(b, b->variant, b->grid, b->stream);
+    if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return set_error(RT_ERR_HIP, "end capture");
+    if (*out) hipGraphExecDestroy(*out);
+    *out = nullptr;
+    hipError_t e = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) return set_error(RT_ERR_HIP, "graph instantiate");
+    return RT_OK;
+}
+
+static int fetch_results(rt_batch* b, rt_pair_result* out) {
+    std::vector<float> tr(16 * (size_t)b->n), rms(b->n);
+    std::vector<int> st(b->n);
+    if (hipMemcpyAsync(tr.data(), b->trel_out, tr.size() * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipMemcpyAsync(rms.data(), b->rms, rms.size() * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipMemcpyAsync(st.data(), b->status, st.size() * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return set_error(RT_ERR_HIP, "result copy");
+    for (int i = 0; i < b->n; i++) {
+        double Td[16], Ti[16];
+        for (int k = 0; k < 16; k++) Td[k] = tr[16 * (size_t)i + k];
+        invert_rt4(Td, Ti);
+        memcpy(out[i].Trel, Td, sizeof(Td));
+        memcpy(out[i].T, Ti, sizeof(Ti));
+        out[i].rms = rms[i];
+        out[i].status = st[i];
+    }
+    return RT_OK;
+}
+
+static int run_track(rt_batch* b, rt_pair_result* out) {
+    int rc = autotune(b);
+    if (rc) return rc;
+    if (!b->graph_track || b->graph_n != b->n || b->graph_variant != b->variant || b->graph_grid != b->grid) {
+        if ((rc = capture(b, &b->graph_track, false))) return rc;
+        b->graph_n = b->n; b->graph_variant = b->variant; b->graph_grid = b->grid;
+    }
+    if (hipGraphLaunch(b->graph_track, b->stream) != hipSuccess) return set_error(RT_ERR_HIP, "graph launch");
+    return out ? fetch_results(b, out) : RT_OK;
+}
+
+extern "C" int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_result* out) {
+    if (!b || !pairs || n < 1 || n > b->max_pairs) return set_error(RT_ERR_ARG, "rt_batch_track: bad argument");
+    if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    std::vector<PairDesc> desc(n);
+    std::vector<float> calib(96 * (size_t)n);
+    int maxW = 0, maxH = 0, maxN = 0;
+    for (int i = 0; i < n; i++) {
+        const rt_pair& p = pairs[i];
+        rt_params q = b->prm;
+        int L = q.levels > 0 ? q.levels : (p.width == 640 ? 4 : 3);
+        int m = std::max(4, 1 << (L - 1));
+        if (!p.ref_gray || !p.ref_depth || !p.cur_gray || !p.cur_depth || !p.calib) return set_error(RT_ERR_ARG, "null pair input");
+        if (p.width > b->max_w || p.height > b->max_h || p.width % (2 * m) || p.height % m || p.width < 32 || p.height < 32)
+            return set_error(RT_ERR_ARG, "pair size not supported by this batch");
+        if ((p.width * p.height) / 512 > 1024) return set_error(RT_ERR_ARG, "pair too large");
+        int N = q.n_select;
+        if (N > p.width * p.height / 2) return set_error(RT_ERR_ARG, "n_select above half the pixels of a pair");
+        PairDesc& d = desc[i];
+        memset(&d, 0, sizeof(d));
+        d.W = p.width; d.H = p.height; d.L = L; d.N = N;
+        for (int k = 0; k < 4; k++) d.iters[k] = q.iters[k];
+        d.calib = b->calib_dev + 96 * (size_t)i;
+        memcpy(&calib[96 * (size_t)i], p.calib, 96 * 4);
+        d.g8[0] = p.ref_gray; d.d16[0] = p.ref_depth; d.g8[1] = p.cur_gray; d.d16[1] = p.cur_depth;
+        char* base = b->arena + b->pair_bytes * (size_t)i;
+        size_t npx = (size_t)p.width * p.height, off = 0;
+        for (int f = 0; f < 2; f++) {
+            size_t lo = 0;
+            for (int l = 0; l < 4; l++) {
+                d.gray[f][l] = l < L ? (float*)(base + off + lo) : nullptr;
+                lo += (npx >> (2 * l)) * 4;
+            }
+            off += align256((size_t)b->max_w * b->max_h * 4 * 4 / 3 + 64);
+        }
+        for (int f = 0; f < 2; f++) { d.z[f] = (float*)(base + off); off += align256((size_t)b->max_w * b->max_h * 4); }
+        d.gmag = (float*)(base + off); off += align256((size_t)b->max_w * b->max_h * 4);
+        d.ghist = (unsigned*)(base + off); off += align256(256 * 4);
+        d.seg = (int*)(base + off); off += align256(2600 * 4);
+        size_t NN = (size_t)b->prm.n_select;
+        d.sel = (int*)(base + off); off += align256(NN * 4);
+        d.pts = (float4*)(base + off); off += align256(NN * 16);
+        for (int l = 0; l < 4; l++) { d.jac[l] = (float4*)(base + off); off += align256(NN * 32); }
+        d.trel_in = b->trel_in + 16 * (size_t)i;
+        d.trel_out = b->trel_out + 16 * (size_t)i;
+        d.rms = b->rms + i;
+        d.status = b->status + i;
+        for (int k = 0; k < 16; k++) b->trel_in_host[16 * (size_t)i + k] = p.init_T ? (float)p.init_T[k] : ((k % 5 == 0) ? 1.0f : 0.0f);
+        maxW = std::max(maxW, d.W); maxH = std::max(maxH, d.H); maxN = std::max(maxN, d.N);
+    }
+    // work queue order: decreasing cost (points x iterations), ties by index
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = i;
+    auto cost = [&](int i) { long c = 0; for (int k = 0; k < desc[i].L; k++) c += desc[i].iters[k]; return c * desc[i].N + (long)desc[i].W * desc[i].H / 8; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    if (hipMemcpyAsync(b->desc_dev, desc.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->calib_dev, calib.data(), calib.size() * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->order_dev, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
+        hipMemcpyAsync(b->trel_in, b->trel_in_host, 64 * (size_t)n, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+        return set_error(RT_ERR_HIP, "descriptor upload");
+    bool changed = b->n != n || b->maxW != maxW || b->maxH != maxH || b->maxN != maxN;
+    b->n = n; b->desc = desc; b->maxW = maxW; b->maxH = maxH; b->maxN = maxN;
+    if (changed || !b->graph_prep) {
+        int rc = capture(b, &b->graph_prep, true);
+        if (rc) return rc;
+    }
+    if (hipGraphLaunch(b->graph_prep, b->stream) != hipSuccess) return set_error(RT_ERR_HIP, "graph launch");
+    return run_track(b, out);
+}
+
+// out == NULL: enqueue the launch and return without waiting (timing).
+extern "C" int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_result* out) {
+    if (!b || b->n < 1) return set_error(RT_ERR_STATE, "rt_batch_retrack: no prepared batch");
+    if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    if (init_T) {
+        for (int i = 0; i < b->n; i++)
+            for (int k = 0; k < 16; k++) b->trel_in_host[16 * (size_t)i + k] = (float)init_T[16 * (size_t)i + k];
+        if (hipMemcpyAsync(b->trel_in, b->trel_in_host, 64 * (size_t)b->n, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+            return set_error(RT_ERR_HIP, "pose upload");
+    }
+    return run_track(b, out);
+}

@@ -1,0 +1,105 @@
+"""Minimal HIP runtime plumbing over ctypes (device buffers, copies, events,
+streams) for the benchmark and tests, so they need not import torch."""
+import ctypes
+import os
+
+import numpy as np
+
+_hip = None
+
+
+def hip():
+    global _hip
+    if _hip is None:
+        for name in ("libamdhip64.so", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                _hip = ctypes.CDLL(name)
+                break
+            except OSError:
+                continue
+        if _hip is None:
+            raise RuntimeError("libamdhip64.so not found")
+        _hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+        _hip.hipFree.argtypes = [ctypes.c_void_p]
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+        _hip.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        _hip.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        _hip.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        _hip.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        _hip.hipEventDestroy.argtypes = [ctypes.c_void_p]
+        _hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+        _hip.hipGetDeviceCount.argtypes = [ctypes.POINTER(ctypes.c_int)]
+        _hip.hipSetDevice.argtypes = [ctypes.c_int]
+        _hip.hipGetErrorString.restype = ctypes.c_char_p
+        _hip.hipGetErrorString.argtypes = [ctypes.c_int]
+    return _hip
+
+
+def _ok(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what}: {hip().hipGetErrorString(rc).decode()}")
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    rc = hip().hipGetDeviceCount(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class DeviceBuffer:
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _ok(hip().hipMalloc(ctypes.byref(p), max(self.nbytes, 1)), "hipMalloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_array(cls, a):
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        _ok(hip().hipMemcpy(b.ptr, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+        return b
+
+    def to_array(self, dtype, count):
+        out = np.empty(count, dtype)
+        _ok(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2), "hipMemcpy D2H")
+        return out
+
+    def free(self):
+        if self.ptr:
+            hip().hipFree(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = ctypes.c_void_p()
+        _ok(hip().hipEventCreate(ctypes.byref(e)), "hipEventCreate")
+        self.e = e.value
+
+    def record(self, stream):
+        _ok(hip().hipEventRecord(self.e, stream), "hipEventRecord")
+
+    def synchronize(self):
+        _ok(hip().hipEventSynchronize(self.e), "hipEventSynchronize")
+
+    def elapsed_ms(self, end):
+        t = ctypes.c_float()
+        _ok(hip().hipEventElapsedTime(ctypes.byref(t), self.e, end.e), "hipEventElapsedTime")
+        return t.value
+
+    def __del__(self):
+        try:
+            hip().hipEventDestroy(self.e)
+        except Exception:
+            pass
+
+
+def stream_sync(stream):
+    _ok(hip().hipStreamSynchronize(stream), "hipStreamSynchronize")

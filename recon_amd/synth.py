@@ -106,5 +106,8 @@ def make_sequence(seed, n, width, height, rot_max_deg, trans_max_mm, noise=0.01,
 def pose_error(T_est, T_gt):
     """(rotation error in degrees, translation error in mm) of two 4x4 poses."""
     D = np.linalg.inv(np.asarray(T_gt, np.float64)) @ np.asarray(T_est, np.float64)
-    c = np.clip((np.trace(D[:3, :3]) - 1.0) / 2.0, -1.0, 1.0)
-    return float(np.degrees(np.arccos(c))), float(np.linalg.norm(D[:3, 3]))
+    R = D[:3, :3]
+    # angle from both the skew part (sin) and the trace (cos): well conditioned near zero
+    s = 0.5 * np.linalg.norm([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
+    c = (np.trace(R) - 1.0) / 2.0
+    return float(np.degrees(np.arctan2(s, c))), float(np.linalg.norm(D[:3, 3]))

@@ -233,7 +233,15 @@ class BatchTracker:
 
     def track(self, pairs, init=None):
         """pairs: list of dicts with device pointers ref_gray, ref_depth, cur_gray,
-        cur_depth (ints), width, height, calib (np.float32[96])."""
+        cur_depth (ints), width, height, calib (np.float32[96]); or the result
+        of prepare()."""
+        prep = pairs if isinstance(pairs, tuple) else self.prepare(pairs, init)
+        arr, n, out, _keep = prep
+        check(self.L.rt_batch_track(self.h, n, arr, out), "rt_batch_track")
+        return _results(out)
+
+    def prepare(self, pairs, init=None):
+        """The ctypes pair array for track(), built once for repeated calls."""
         n = len(pairs)
         arr = (RtPair * n)()
         keep = []
@@ -249,8 +257,7 @@ class BatchTracker:
                 keep.append(t)
                 arr[i].init_T = t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         out = (RtPairResult * n)()
-        check(self.L.rt_batch_track(self.h, n, arr, out), "rt_batch_track")
-        return _results(out)
+        return (arr, n, out, keep)
 
     def retrack(self, init=None, n=None):
         out = (RtPairResult * n)()
