@@ -16,10 +16,11 @@ _lib = None
 _ref = None
 
 
-def build(with_reference=False):
-    subprocess.run(["make", "-s", "-C", HERE, "all"], check=True)
+def build(with_reference=False, force=False):
+    flags = ["-B"] if force else []
+    subprocess.run(["make", "-s"] + flags + ["-C", HERE, "all"], check=True)
     if with_reference and os.path.isdir("/root/reference/tracker"):
-        subprocess.run(["make", "-s", "-C", HERE, "ref"], check=True)
+        subprocess.run(["make", "-s"] + flags + ["-C", HERE, "ref"], check=True)
 
 
 def lib():

@@ -89,6 +89,15 @@ void or_expm_update_d(const double* x, double* T, double scale) { expm_update<do
 void or_apply_increment(const float* Trel, float* T, double* Tinc, double* global) {
     apply_increment(Trel, T, Tinc, global);
 }
+void or_rot2quat(const float* m, float* q, float* qn) {
+    rot2quat(m, q);
+    float len = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3] + 1e-7f;
+    len = (float)std::sqrt((double)len);
+    for (int i = 0; i < 4; i++) qn[i] = q[i] / len;
+}
+void or_invert_rt4_d(const double* M, double* Mi) { invert_rt4<double>(M, Mi); }
+void or_invert_rt4_f(const float* M, float* Mi) { invert_rt4<float>(M, Mi); }
+void or_mat4_mul(const double* A, const double* B, double* C) { mat4_mul(A, B, C); }
 int or_pose_difference_exceeded(const float* dT, float tthr, float athr) {
     return pose_difference_exceeded(dT, tthr, athr) ? 1 : 0;
 }
