@@ -21,7 +21,9 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
-BYTES_PER_POINT_ITER = 60  # per selected point per GN iteration (point 16 + J 24 + 4 taps 16 + depth tap 4)
+# Algorithmic bytes per selected point per Gauss-Newton iteration: keyframe point
+# xyz 12 + reference gray 4 + Jacobian 24 + four bilinear gray taps 16 + depth tap 4.
+BYTES_PER_POINT_ITER = 60
 
 
 def log(*a):
@@ -68,6 +70,73 @@ def pose_errors(Trel_est, Trel_gt):
     return e[:, 0], e[:, 1]
 
 
+def measured_traffic(W, H, B):
+    """HBM bytes per launch of the tracking kernel at this workload, from the
+    committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE scaled by a
+    copy kernel of known size); None where no summary exists for it."""
+    path = os.path.join(ROOT, "profiles", f"kb_track_{W}x{H}_{B}pairs.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return int(float(line.split(":")[1]))
+    return None
+
+
+def copy_probe(mib):
+    """One device-to-device copy kernel of a known size: reads and writes
+    mib MiB each, so per-dispatch FETCH_SIZE / WRITE_SIZE of this kernel give
+    the counters' scale in bytes."""
+    import torch
+    n = mib << 20
+    x = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    x.fill_(7)
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    assert int(y[n - 1]) == 7
+    log(f"copy probe: {n} bytes copied")
+
+
+def cpu_baseline(g, d, npairs, W, H, L, N, iters, cal, seconds, threads):
+    """The oracle's FP32 tracker over a sample of the same pairs, one pair per
+    thread at a time, for about `seconds` of wall time; frames/s over all
+    threads."""
+    import threading
+
+    import oracle
+    npx = W * H
+    gh = g.to_array(np.uint8, 2 * npairs * npx).reshape(2 * npairs, H, W)
+    dh = d.to_array(np.uint16, 2 * npairs * npx).reshape(2 * npairs, H, W)
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    done = [0] * threads
+    t0 = time.perf_counter()
+    stop = t0 + seconds
+
+    def work(k):
+        i = k
+        while time.perf_counter() < stop:
+            p = i % npairs
+            o = oracle.Tracker(W, H, cal, mode=oracle.PHOTOMETRIC, levels=L, iters=iters, nselect=N, dtype=np.float32)
+            o.set_reference(gh[2 * p], dh[2 * p])
+            o.set_current(gh[2 * p + 1], dh[2 * p + 1])
+            o.track()
+            done[k] += 1
+            i += threads
+
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    el = time.perf_counter() - t0
+    n = sum(done)
+    return {"value": round(n / el, 2), "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{n} pairs cycled from the first {npairs} {W}x{H} pairs of the batch, FP32 oracle tracker, "
+                      f"{threads} threads, {el:.1f} s wall"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,9 +148,18 @@ def main():
     ap.add_argument("--height", type=int, default=480)
     ap.add_argument("--theta", type=float, default=3.0, help="max rotation (deg) of a pair")
     ap.add_argument("--tmax", type=float, default=50.0, help="max translation (mm) of a pair")
-    ap.add_argument("--cpu-pairs", type=int, default=8, help="pairs timed on the CPU baseline (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall time of the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--track-only", action="store_true",
+                    help="prepare the batch and run --steps tracking launches only (for counter collection)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="run one device copy of MIB MiB and exit (calibrates HBM byte counters)")
     args = ap.parse_args()
+
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -102,6 +180,18 @@ def main():
     log(f"inputs: {B} pairs {W}x{H} rendered in {time.time() - t0:.1f}s")
     npx = W * H
     cal = calib_block(W, H)
+
+    if args.track_only:
+        bt = BatchTracker(B, W, H, mode=PHOTOMETRIC, n_select=N, iters=iters)
+        pairs = [dict(ref_gray=g.ptr + 2 * i * npx, ref_depth=d.ptr + 2 * i * npx * 2,
+                      cur_gray=g.ptr + (2 * i + 1) * npx, cur_depth=d.ptr + (2 * i + 1) * npx * 2,
+                      width=W, height=H, calib=cal) for i in range(B)]
+        bt.track(bt.prepare(pairs))
+        for _ in range(args.steps):
+            bt.L.rt_batch_retrack(bt.h, None, None)
+        stream_sync(bt.L.rt_batch_stream(bt.h))
+        log(f"track-only: {args.steps} tracking launches over {B} pairs done")
+        return
 
     # ---- plain single-pair tracker (same kernels, one pair at a time)
     P = min(args.plain_pairs, B)
@@ -162,23 +252,11 @@ def main():
     log(f"pose error vs truth: batched median {np.median(rb):.4f} deg {np.median(tb):.3f} mm, "
         f"plain median {np.median(rp):.4f} deg {np.median(tp):.3f} mm; batched vs plain max {ra.max():.2e} deg {ta.max():.2e} mm")
 
-    # ---- CPU baseline: the oracle's FP32 transcription on the host, single thread
+    # ---- CPU baseline: the oracle's FP32 transcription on the host cores
     cpu = None
-    if args.cpu_pairs > 0:
-        import oracle
-        from recon_amd.hip import hip
-        gh = g.to_array(np.uint8, 2 * args.cpu_pairs * npx)
-        dh = d.to_array(np.uint16, 2 * args.cpu_pairs * npx)
-        t3 = time.perf_counter()
-        for i in range(args.cpu_pairs):
-            o = oracle.Tracker(W, H, cal, mode=oracle.PHOTOMETRIC, levels=L, iters=iters, nselect=N, dtype=np.float32)
-            o.set_reference(gh[2 * i * npx:(2 * i + 1) * npx].reshape(H, W), dh[2 * i * npx:(2 * i + 1) * npx].reshape(H, W))
-            o.set_current(gh[(2 * i + 1) * npx:(2 * i + 2) * npx].reshape(H, W), dh[(2 * i + 1) * npx:(2 * i + 2) * npx].reshape(H, W))
-            o.track()
-        cpu_s = time.perf_counter() - t3
-        cpu = {"value": round(args.cpu_pairs / cpu_s, 3), "unit": "frames/s", "cores": 1, "kind": "port",
-               "sample": f"{args.cpu_pairs} of the {W}x{H} pairs, FP32 oracle transcription, one thread ({cpu_s:.1f} s)"}
-        log(f"cpu baseline: {cpu['value']} frames/s")
+    if args.cpu_seconds > 0:
+        cpu = cpu_baseline(g, d, min(B, 64), W, H, L, N, iters, cal, args.cpu_seconds, args.cpu_threads)
+        log(f"cpu baseline: {cpu['value']} frames/s ({cpu['sample']})")
 
     out = {
         "metric": "tracked frames/s at 640x480 (learned+batched vs plain GN) at matched pose error",
@@ -202,7 +280,8 @@ def main():
                        "plain_median_deg": float(np.median(rp)), "plain_median_mm": float(np.median(tp)),
                        "batched_vs_plain_max_deg": float(ra.max()), "batched_vs_plain_max_mm": float(ta.max())},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(W, H, B),
+                     "algorithmic_bytes": alg_bytes,
                      "kernel": "kb_track (persistent Gauss-Newton)", "kernel_ms": round(k_ms, 3)},
         "cpu_baseline": cpu,
     }

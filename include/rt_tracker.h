@@ -211,7 +211,7 @@ enum {
     RT_DBG_GRAD_HIST = 5,    /* uint32 256 */
     RT_DBG_SELECTED = 6,     /* int32 n_select */
     RT_DBG_POINTS = 7,       /* float4 n_select: x, y, z, 0 */
-    RT_DBG_JACOBIAN = 8,     /* float 8*n_select per level: J0..J5, I_ref, 0 per point */
+    RT_DBG_JACOBIAN = 8,     /* per level, n_select points: float4 {J0..J3}, then float2 {J4,J5}, then float I_ref */
     RT_DBG_RESIDUAL = 9,     /* float2 n_select: r, zWeight (last photometric iteration) */
     RT_DBG_TREL = 10,        /* float 16 */
     RT_DBG_REF_CLOUD = 11,   /* float4 2*W_l*H_l: (x,y,z,mask),(nx,ny,nz,0) per pixel */

@@ -36,7 +36,7 @@ struct PairDesc {
     const uint16_t* d16[2];
     float* gray[2][4];
     float* z[2];
-    float* gmag;
+    uint8_t* bins;          // per-pixel gradient-magnitude bin
     unsigned* ghist;
     int* seg;
     int* sel;
@@ -49,35 +49,117 @@ struct PairDesc {
 };
 
 // ------------------------------------------------------------------ batched preparation
-__global__ __launch_bounds__(256) void kb_convert(const PairDesc* __restrict__ P, int which) {
+// Level 0 of both frames stays in its input form (8-bit gray, 16-bit depth):
+// every consumer reads it through the exact conversions (U8Src, U16DepthSrc),
+// so no FP32 copy is written or read back.
+__device__ __forceinline__ void load_lut(float* lut) {
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) lut[t] = (float)t / 255.0f;
+}
+
+// pyramid levels 1..L-1 of one frame (fastDownSampleHdrKernel, downSample2Cuda.cu:35-42)
+__global__ __launch_bounds__(256) void kb_pyramid(const PairDesc* __restrict__ P, int which) {
     const PairDesc& d = P[blockIdx.z];
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 64 >= d.H) return;
-    convert_pyramid_block<0>(blockIdx.x, blockIdx.y, d.g8[which], d.d16[which], nullptr, d.W, d.H, d.L, d.calib,
-                             d.gray[which][0], d.gray[which][1], d.gray[which][2], d.gray[which][3], d.z[which]);
+    convert_pyramid_block<3>(blockIdx.x, blockIdx.y, d.g8[which], nullptr, nullptr, d.W, d.H, d.L, d.calib, nullptr,
+                             d.gray[which][1], d.gray[which][2], d.gray[which][3], nullptr);
     if (which == 0 && blockIdx.x == 0 && blockIdx.y == 0) d.ghist[threadIdx.x] = 0u;
 }
-__global__ __launch_bounds__(256) void kb_features(const PairDesc* __restrict__ P) {
+
+// xyz2CloudKernel's gradient magnitude (d2z.cu:235-331) as an 8-bit bin per
+// pixel, and the 256-bin histogram (cuda_hist.cu:100-152). Tiles of 64x4: one
+// wave per image row segment; a wave adds each distinct bin it holds once.
+__global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.z];
-    if ((int)blockIdx.x * 16 >= d.W || (int)blockIdx.y * 16 >= d.H) return;
-    ref_features_block(blockIdx.x, blockIdx.y, d.z[0], d.gray[0][0], d.W, d.H, d.calib, d.gmag, d.ghist);
+    if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 4 >= d.H) return;
+    __shared__ float lut[256];
+    __shared__ unsigned h[256];
+    load_lut(lut);
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    int lane = lane_id();
+    int x = blockIdx.x * 64 + lane, y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    bool in = x < d.W && y < d.H;
+    unsigned b = 0;
+    if (in) {
+        PixFeat f = pixel_features(U16DepthSrc{d.d16[0]}, U8Src{d.g8[0], lut}, d.W, d.H, x, y, d.calib);
+        b = grad_bin(f.gmag);
+        d.bins[(size_t)x + (size_t)y * d.W] = (uint8_t)b;
+    }
+    uint64_t active = __ballot(in);
+    while (active) {
+        int leader = __ffsll((unsigned long long)active) - 1;
+        unsigned bl = __shfl(b, leader, 64);
+        uint64_t m = __ballot(in && b == bl);
+        if (lane == leader) atomicAdd(&h[bl], (unsigned)__popcll(m));
+        active &= ~m;
+    }
+    __syncthreads();
+    unsigned v = h[threadIdx.x];
+    if (v) atomicAdd(&d.ghist[threadIdx.x], v);
 }
+
+// filterIndexKernel4 (filterIndex256.cu:110-169) pass 1 with one wave per
+// segment: counts of bins above and equal to the threshold.
 __global__ __launch_bounds__(256) void kb_select_count(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.y];
-    select_count_block(blockIdx.x, d.gmag, d.W * d.H, d.ghist, d.N, d.seg);
+    int thr, excess;
+    find_threshold(d.ghist, d.N, thr, excess);
+    int wave = threadIdx.x >> 6, lane = lane_id();
+    int s = blockIdx.x * 4 + wave;
+    int npix = d.W * d.H, Pn = npix / 512, b0 = s * Pn;
+    int cA = 0, cE = 0;
+    for (int off = 0; off < Pn; off += 64) {
+        bool in = off + lane < Pn;
+        unsigned bn = in ? d.bins[b0 + off + lane] : 0u;
+        cA += __popcll(__ballot(in && bn > (unsigned)thr));
+        cE += __popcll(__ballot(in && bn == (unsigned)thr));
+    }
+    if (lane == 0) {
+        d.seg[s] = cA;
+        d.seg[512 + s] = cE;
+        if (s == 0) { d.seg[2560] = thr; d.seg[2561] = excess; }
+    }
 }
 __global__ __launch_bounds__(512) void kb_select_scan(const PairDesc* __restrict__ P) {
     select_scan_block(P[blockIdx.x].seg);
 }
+// pass 3: ordered scatter, one wave per segment
 __global__ __launch_bounds__(256) void kb_select_scatter(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.y];
-    select_scatter_block(blockIdx.x, d.gmag, d.W * d.H, d.N, d.seg, d.sel);
+    int wave = threadIdx.x >> 6, lane = lane_id();
+    int s = blockIdx.x * 4 + wave;
+    const int* seg = d.seg;
+    unsigned thr = (unsigned)seg[2560];
+    int startA = seg[1024 + s], startE = seg[1536 + s], mE = seg[2048 + s];
+    int npix = d.W * d.H, Pn = npix / 512, b0 = s * Pn, N = d.N;
+    uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    int rA = 0, rE = 0;
+    for (int off = 0; off < Pn; off += 64) {
+        int i = b0 + off + lane;
+        bool in = off + lane < Pn;
+        unsigned bn = in ? d.bins[i] : 0u;
+        bool isA = in && bn > thr, isE = in && bn == thr;
+        uint64_t ma = __ballot(isA), me = __ballot(isE);
+        if (isA) { int pos = startA + rA + __popcll(ma & lt); if (pos < N) d.sel[pos] = i; }
+        int e = rE + __popcll(me & lt);
+        if (isE && e < mE) { int pos = startE + e; if (pos < N) d.sel[pos] = i; }
+        rA += __popcll(ma);
+        rE += __popcll(me);
+    }
+    if (s == 0) {
+        int total = seg[2563];
+        for (int k = (total > 0 ? total : 0) + lane; k < N; k += 64) d.sel[k] = 0;
+    }
 }
 __global__ __launch_bounds__(256) void kb_ref_points(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.y];
+    __shared__ float lut[256];
+    load_lut(lut);
+    __syncthreads();
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= d.N) return;
-    ref_point(i, d.z[0], d.gray[0][0], d.gray[0][1], d.gray[0][2], d.gray[0][3], d.W, d.H, d.L, d.N, d.calib, 1e-3f,
-              d.sel, d.pts, d.jac[0], d.jac[1], d.jac[2], d.jac[3]);
+    ref_point(i, U16DepthSrc{d.d16[0]}, U8Src{d.g8[0], lut}, d.gray[0][1], d.gray[0][2], d.gray[0][3], d.W, d.H, d.L,
+              d.N, d.calib, 1e-3f, d.sel, d.pts, d.jac[0], d.jac[1], d.jac[2], d.jac[3]);
 }
 __global__ void kb_reset_queue(int* q) {
     if (threadIdx.x == 0) q[0] = 0;
@@ -101,7 +183,9 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
     __shared__ double wsum[NW][29];
     __shared__ int s_pair, s_med;
     __shared__ int s_ok;
+    __shared__ float lut[256];
     const int t = threadIdx.x, wave = t >> 6, lane = lane_id();
+    load_lut(lut);
     for (;;) {
         __syncthreads();
         if (t == 0) {
@@ -121,7 +205,7 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
         for (int layer = L - 1; layer >= 0; layer--) {
             const int nit = d.iters[L - 1 - layer];
             const float* img = d.gray[1][layer];
-            const float4* __restrict__ jac = d.jac[layer];
+            const JacView jv = jac_view(d.jac[layer], N);
             const int wl = W >> layer, hl = H >> layer;
             const float a = 1.0f / (float)(1 << layer), b = 0.5f * (a - 1.0f);
             for (int it = 0; it < nit; it++) {
@@ -137,7 +221,12 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
                     int i = t + k * BS;
                     r[k] = 0.0f; zw[k] = 0.0f;
                     if (i < N) {
-                        photo_point(d.pts[i], jac[2 * i + 1].z, Tl, a, b, img, wl, hl, d.z[1], W, H, cal, r[k], zw[k]);
+                        if (layer == 0)
+                            photo_point(d.pts[i], jv.iref[i], Tl, a, b, U8Src{d.g8[1], lut}, wl, hl,
+                                        U16DepthSrc{d.d16[1]}, W, H, cal, r[k], zw[k]);
+                        else
+                            photo_point(d.pts[i], jv.iref[i], Tl, a, b, F32Src{img}, wl, hl,
+                                        U16DepthSrc{d.d16[1]}, W, H, cal, r[k], zw[k]);
                         int bn = (int)((float)(cfg.mad_bins - 1) * fabsf(r[k]));
                         if (bn >= 0 && bn < cfg.mad_bins) atomicAdd(&hist[bn], 1u);
                     }
@@ -178,7 +267,8 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
                     if (i < N) {
                         float w = robust_weight(r[k], zw[k], s, cfg.robust);
                         float wr = r[k] * w;
-                        float4 ja = jac[2 * i], jb = jac[2 * i + 1];
+                        float4 ja = jv.a[i];
+                        float2 jb = jv.b[i];
                         float jw[6] = {ja.x * w, ja.y * w, ja.z * w, ja.w * w, jb.x * w, jb.y * w};
                         int q = 0;
 #pragma unroll
@@ -285,12 +375,10 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return set_error(RT_ERR_HIP, "stream"); }
     size_t npx = (size_t)max_w * max_h;
     size_t N = (size_t)b->prm.n_select;
-    (void)npx;
     size_t per = 0;  // inputs are borrowed device pointers; the arena holds derived data only
-    per += 2 * align256(npx * 4 * 4 / 3 + 64);  // two gray pyramids
-    per += 2 * align256(npx * 4);               // two depth maps
-    per += align256(npx * 4);                   // gradient magnitudes
-    per += align256(256 * 4) + align256(2600 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(N * 32);
+    per += 2 * align256(npx * 4 / 3 + 64);      // two gray pyramids, levels 1..3
+    per += align256(npx);                       // gradient-magnitude bins
+    per += align256(256 * 4) + align256(2600 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(jac_bytes((int)N));
     b->pair_bytes = per;
     size_t total = per * (size_t)max_pairs;
     void* q = nullptr;
@@ -333,13 +421,13 @@ extern "C" void* rt_batch_stream(rt_batch* b) { return b ? (void*)b->stream : nu
 static void launch_prep(rt_batch* b, hipStream_t s) {
     int n = b->n;
     dim3 g64((b->maxW + 63) / 64, (b->maxH + 63) / 64, n);
-    kb_convert<<<g64, 256, 0, s>>>(b->desc_dev, 0);
-    kb_convert<<<g64, 256, 0, s>>>(b->desc_dev, 1);
-    dim3 g16((b->maxW + 15) / 16, (b->maxH + 15) / 16, n);
-    kb_features<<<g16, 256, 0, s>>>(b->desc_dev);
-    kb_select_count<<<dim3(512, n), 256, 0, s>>>(b->desc_dev);
+    kb_pyramid<<<g64, 256, 0, s>>>(b->desc_dev, 0);
+    kb_pyramid<<<g64, 256, 0, s>>>(b->desc_dev, 1);
+    dim3 gb((b->maxW + 63) / 64, (b->maxH + 3) / 4, n);
+    kb_bins<<<gb, 256, 0, s>>>(b->desc_dev);
+    kb_select_count<<<dim3(128, n), 256, 0, s>>>(b->desc_dev);
     kb_select_scan<<<n, 512, 0, s>>>(b->desc_dev);
-    kb_select_scatter<<<dim3(512, n), 256, 0, s>>>(b->desc_dev);
+    kb_select_scatter<<<dim3(128, n), 256, 0, s>>>(b->desc_dev);
     kb_ref_points<<<dim3((b->maxN + 255) / 256, n), 256, 0, s>>>(b->desc_dev);
 }
 
@@ -479,20 +567,21 @@ extern "C" int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_
         size_t npx = (size_t)p.width * p.height, off = 0;
         for (int f = 0; f < 2; f++) {
             size_t lo = 0;
-            for (int l = 0; l < 4; l++) {
+            d.gray[f][0] = nullptr;  // level 0 is read from the 8-bit input
+            for (int l = 1; l < 4; l++) {
                 d.gray[f][l] = l < L ? (float*)(base + off + lo) : nullptr;
                 lo += (npx >> (2 * l)) * 4;
             }
-            off += align256((size_t)b->max_w * b->max_h * 4 * 4 / 3 + 64);
+            off += align256((size_t)b->max_w * b->max_h * 4 / 3 + 64);
         }
-        for (int f = 0; f < 2; f++) { d.z[f] = (float*)(base + off); off += align256((size_t)b->max_w * b->max_h * 4); }
-        d.gmag = (float*)(base + off); off += align256((size_t)b->max_w * b->max_h * 4);
+        d.z[0] = d.z[1] = nullptr;   // depth is read from the 16-bit input
+        d.bins = (uint8_t*)(base + off); off += align256((size_t)b->max_w * b->max_h);
         d.ghist = (unsigned*)(base + off); off += align256(256 * 4);
         d.seg = (int*)(base + off); off += align256(2600 * 4);
         size_t NN = (size_t)b->prm.n_select;
         d.sel = (int*)(base + off); off += align256(NN * 4);
         d.pts = (float4*)(base + off); off += align256(NN * 16);
-        for (int l = 0; l < 4; l++) { d.jac[l] = (float4*)(base + off); off += align256(NN * 32); }
+        for (int l = 0; l < 4; l++) { d.jac[l] = (float4*)(base + off); off += align256(jac_bytes((int)NN)); }
         d.trel_in = b->trel_in + 16 * (size_t)i;
         d.trel_out = b->trel_out + 16 * (size_t)i;
         d.rms = b->rms + i;

@@ -57,13 +57,34 @@ __device__ __forceinline__ f2 distort(f2 pu, const float* kc, const float* K) {
 __device__ __forceinline__ bool in_bounds(f2 p, int w, int h) {
     return p.x >= 10.0f && p.x < (float)(w - 10) && p.y >= 10.0f && p.y < (float)(h - 10);
 }
+// Image sources: FP32 planes, or the raw inputs read through their exact
+// conversions: 8-bit gray as (float)g/255 from a 256-entry table, 16-bit depth
+// in mm as -(float)d (the values the conversion kernel would have stored).
+struct F32Src {
+    const float* __restrict__ p;
+    __device__ __forceinline__ float operator[](size_t i) const { return p[i]; }
+};
+struct U8Src {
+    const uint8_t* __restrict__ p;
+    const float* lut;
+    __device__ __forceinline__ float operator[](size_t i) const { return lut[p[i]]; }
+};
+struct U16DepthSrc {
+    const uint16_t* __restrict__ p;
+    __device__ __forceinline__ float operator[](size_t i) const { return -(float)p[i]; }
+};
+
 // bilinearInterpolation(xdi,ydi,fx,fy,...) (kernelUtils.h:267-272)
 __device__ __forceinline__ float blend(float v0, float v1, float v2, float v3, float fx, float fy) {
     return (1.0f - fx) * (1.0f - fy) * v0 + fx * (1.0f - fy) * v1 + (1.0f - fx) * fy * v2 + fx * fy * v3;
 }
-__device__ __forceinline__ float bilerp(const float* __restrict__ img, int width, int xdi, int ydi, float fx, float fy) {
-    int o = xdi + ydi * width;
+template <class S>
+__device__ __forceinline__ float bilerp(const S& img, int width, int xdi, int ydi, float fx, float fy) {
+    size_t o = (size_t)xdi + (size_t)ydi * width;
     return blend(img[o], img[o + 1], img[o + width], img[o + width + 1], fx, fy);
+}
+__device__ __forceinline__ float bilerp(const float* __restrict__ img, int width, int xdi, int ydi, float fx, float fy) {
+    return bilerp(F32Src{img}, width, xdi, ydi, fx, fy);
 }
 // The same with every tap clamped into the image: used where the reference's
 // taps can leave the image (the coarsest attribute level near the border).
@@ -93,11 +114,12 @@ __device__ __forceinline__ bool backproject(float z, int xi, int yi, const float
 // a valid projection gets the reference's empty record: point (0,0,-maxDist),
 // zero gray and gradient.
 struct PixFeat { f3 p; float c1, gx, gy, gmag; bool ok; };
-__device__ __forceinline__ PixFeat pixel_features(const float* __restrict__ z, const float* __restrict__ g0, int W,
-                                                  int H, int xi, int yi, const float* c) {
+template <class ZS, class GS>
+__device__ __forceinline__ PixFeat pixel_features(const ZS& z, const GS& g0, int W, int H, int xi, int yi,
+                                                  const float* c) {
     PixFeat f;
     f3 p3;
-    backproject(z[xi + yi * W], xi, yi, c, p3);
+    backproject(z[(size_t)xi + (size_t)yi * W], xi, yi, c, p3);
     if (p3.z < 0.0f) {
         f3 r3 = mul4(c + TLR, p3);
         f2 pu{r3.x / r3.z, r3.y / r3.z};
@@ -105,11 +127,21 @@ __device__ __forceinline__ PixFeat pixel_features(const float* __restrict__ z, c
         if (in_bounds(p, W, H)) {
             int xdi = (int)p.x, ydi = (int)p.y;
             float fx = p.x - (float)xdi, fy = p.y - (float)ydi;
-            float c1 = bilerp(g0, W, xdi, ydi, fx, fy);
-            float cN = bilerp(g0, W, (int)p.x, (int)(p.y - 1.0f), fx, fy);
-            float cE = bilerp(g0, W, (int)(p.x + 1.0f), (int)p.y, fx, fy);
-            float cW = bilerp(g0, W, (int)(p.x - 1.0f), (int)p.y, fx, fy);
-            float cS = bilerp(g0, W, (int)p.x, (int)(p.y + 1.0f), fx, fy);
+            // The five samples read a 4x4 neighbourhood; load its 12 used taps once.
+            // The reference indexes the shifted samples as (int)(p +- 1.0f),
+            // which equals xdi +- 1 unless p sits just below a power of two
+            // where the sum rounds up: those samples are read separately.
+            int xe = (int)(p.x + 1.0f), xw = (int)(p.x - 1.0f), yn = (int)(p.y - 1.0f), ys = (int)(p.y + 1.0f);
+            size_t o = (size_t)xdi + (size_t)ydi * W;
+            float a01 = g0[o - W], a02 = g0[o - W + 1];
+            float a10 = g0[o - 1], a11 = g0[o], a12 = g0[o + 1], a13 = g0[o + 2];
+            float a20 = g0[o + W - 1], a21 = g0[o + W], a22 = g0[o + W + 1], a23 = g0[o + W + 2];
+            float a31 = g0[o + 2 * W], a32 = g0[o + 2 * W + 1];
+            float c1 = blend(a11, a12, a21, a22, fx, fy);
+            float cN = yn == ydi - 1 ? blend(a01, a02, a11, a12, fx, fy) : bilerp(g0, W, xdi, yn, fx, fy);
+            float cE = xe == xdi + 1 ? blend(a12, a13, a22, a23, fx, fy) : bilerp(g0, W, xe, ydi, fx, fy);
+            float cW = xw == xdi - 1 ? blend(a10, a11, a20, a21, fx, fy) : bilerp(g0, W, xw, ydi, fx, fy);
+            float cS = ys == ydi + 1 ? blend(a21, a22, a31, a32, fx, fy) : bilerp(g0, W, xdi, ys, fx, fy);
             f.gx = (cE - cW) / 2.0f;
             f.gy = (cS - cN) / 2.0f;
             f.gmag = fminf(fabsf(f.gy) + fabsf(f.gx), 1.0f);
@@ -122,6 +154,10 @@ __device__ __forceinline__ PixFeat pixel_features(const float* __restrict__ z, c
     f.p = f3{0.0f, 0.0f, -c[MAXD]};
     f.c1 = 0.0f; f.gx = 0.0f; f.gy = 0.0f; f.gmag = 0.0f; f.ok = false;
     return f;
+}
+__device__ __forceinline__ PixFeat pixel_features(const float* __restrict__ z, const float* __restrict__ g0, int W,
+                                                  int H, int xi, int yi, const float* c) {
+    return pixel_features(F32Src{z}, F32Src{g0}, W, H, xi, yi, c);
 }
 
 // gradient-magnitude bin (gpu_hist.cu:135, filterIndex256.cu:126), fused multiply-add

@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_ref_points(const float* __restrict__ z,
                                                     float4* __restrict__ j2, float4* __restrict__ j3) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N) return;
-    ref_point(i, z, g0, g1, g2, g3, W, H, L, N, c, scale_in, sel, pts, j0, j1, j2, j3);
+    ref_point(i, F32Src{z}, F32Src{g0}, g1, g2, g3, W, H, L, N, c, scale_in, sel, pts, j0, j1, j2, j3);
 }
 
 void launch_prepare_photo_reference(const Dev& d, hipStream_t s) {
@@ -185,9 +185,10 @@ __global__ __launch_bounds__(256) void k_photo_res(const float4* __restrict__ pt
     __syncthreads();
     float a = 1.0f / (float)(1 << layer);
     float b = 0.5f * (a - 1.0f);
+    JacView jv = jac_view(const_cast<float4*>(jac), N);
     for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
         float ir, zW;
-        photo_point(pts[i], jac[2 * i + 1].z, T, a, b, img, wl, hl, zcur, W, H, c, ir, zW);
+        photo_point(pts[i], jv.iref[i], T, a, b, F32Src{img}, wl, hl, F32Src{zcur}, W, H, c, ir, zW);
         res[i] = make_float2(ir, zW);
         int bn = (int)((float)(bins - 1) * fabsf(ir));
         if (bn >= 0 && bn < bins) atomicAdd(&h[bn], 1u);
@@ -223,13 +224,15 @@ __global__ __launch_bounds__(256) void k_photo_sys(const float2* __restrict__ re
     float acc[27];
 #pragma unroll
     for (int k = 0; k < 27; k++) acc[k] = 0.0f;
+    JacView jv = jac_view(const_cast<float4*>(jac), N);
     for (int i = blockIdx.x * 256 + t; i < N; i += gridDim.x * 256) {
         float2 rz = res[i];
         float r = rz.x;
         float w = robust_weight(r, rz.y, s, robust);
         float wr = r * w;
         wts[i] = w;
-        float4 ja = jac[2 * i], jb = jac[2 * i + 1];
+        float4 ja = jv.a[i];
+        float2 jb = jv.b[i];
         float jw[6] = {ja.x * w, ja.y * w, ja.z * w, ja.w * w, jb.x * w, jb.y * w};
         int k = 0;
 #pragma unroll

@@ -5,6 +5,23 @@
 
 namespace rt {
 
+// Per-level point records, structure of arrays over the N points of a level:
+// {J0,J1,J2,J3} (float4), {J4,J5} (float2), reference gray (float); 28 bytes a
+// point. The residual pass reads only the gray, the system pass only J.
+struct JacView {
+    float4* a;
+    float2* b;
+    float* iref;
+};
+__host__ __device__ inline JacView jac_view(float4* base, int N) {
+    JacView v;
+    v.a = base;
+    v.b = reinterpret_cast<float2*>(base + N);
+    v.iref = reinterpret_cast<float*>(v.b + N);
+    return v;
+}
+inline size_t jac_bytes(int N) { return (size_t)N * 28; }
+
 // Device buffers of one image pyramid.
 struct Pyr {
     float* gray[4];
@@ -23,7 +40,7 @@ struct Dev {
     int* seg;                   // 512 cntA | 512 cntE | 512 startA | 512 startE | 512 mcTrim | 4 (thr, excess, totalA, total)
     int* sel;                   // N
     float4* pts;                // N
-    float4* jac[4];             // 2N per level
+    float4* jac[4];             // per level: jac_view layout, 28 bytes a point
     // photometric iteration
     float2* res;                // N: r, zWeight
     float* wts;                 // N: robust weight of the last iteration

@@ -26,7 +26,11 @@ __device__ __forceinline__ void convert_pyramid_block(int bx, int by, const uint
         for (int r = 0; r < 4; r++) {
             int y = y0 + r;
             size_t o = (size_t)y * W + x0;
-            if (SRC == 0) {
+            if (SRC == 3) {  // 8-bit gray only: level 0 stays in its input form
+                uchar4 gg = *reinterpret_cast<const uchar4*>(g8 + o);
+                v[r][0] = (float)gg.x / 255.0f; v[r][1] = (float)gg.y / 255.0f;
+                v[r][2] = (float)gg.z / 255.0f; v[r][3] = (float)gg.w / 255.0f;
+            } else if (SRC == 0) {
                 uchar4 gg = *reinterpret_cast<const uchar4*>(g8 + o);
                 ushort4 dd = *reinterpret_cast<const ushort4*>(d16 + o);
                 v[r][0] = (float)gg.x / 255.0f; v[r][1] = (float)gg.y / 255.0f;
@@ -214,7 +218,8 @@ __device__ __forceinline__ void select_scatter_block(int s, const float* __restr
 }
 
 
-__device__ __forceinline__ void ref_point(int i, const float* __restrict__ z, const float* __restrict__ g0,
+template <class ZS, class GS>
+__device__ __forceinline__ void ref_point(int i, const ZS& z, const GS& g0,
                                                     const float* __restrict__ g1, const float* __restrict__ g2,
                                                     const float* __restrict__ g3, int W, int H, int L, int N,
                                                     const float* __restrict__ c, float scale_in,
@@ -225,7 +230,7 @@ __device__ __forceinline__ void ref_point(int i, const float* __restrict__ z, co
     int xi = idx % W, yi = idx / W;
     PixFeat f = pixel_features(z, g0, W, H, xi, yi, c);
     float gx[4] = {f.gx, 0.0f, 0.0f, 0.0f}, gy[4] = {f.gy, 0.0f, 0.0f, 0.0f}, col[4] = {f.c1, 0.0f, 0.0f, 0.0f};
-    const float* lv[4] = {g0, g1, g2, g3};
+    const float* lv[4] = {nullptr, g1, g2, g3};
     {
         f3 r3 = mul4(c + TLR, f.p);
         f2 pu{r3.x / r3.z, r3.y / r3.z};
@@ -284,8 +289,10 @@ __device__ __forceinline__ void ref_point(int i, const float* __restrict__ z, co
     pts[i] = make_float4(f.p.x, f.p.y, f.p.z, 0.0f);
     float4* jl[4] = {j0, j1, j2, j3};
     for (int l = 0; l < L; l++) {
-        jl[l][2 * i] = make_float4(J[l][0], J[l][1], J[l][2], J[l][3]);
-        jl[l][2 * i + 1] = make_float4(J[l][4], J[l][5], col[l], 0.0f);
+        JacView v = jac_view(jl[l], N);
+        v.a[i] = make_float4(J[l][0], J[l][1], J[l][2], J[l][3]);
+        v.b[i] = make_float2(J[l][4], J[l][5]);
+        v.iref[i] = col[l];
     }
 }
 
@@ -443,10 +450,10 @@ __device__ void solve_update(const double* sys, float* trel, const SolveCfg& cfg
 // bilerp(I_cur(l)) (1.0 outside the view) and the zWeight from the nearest
 // full-resolution current depth. a = 2^-l, b = (a-1)/2 (interpolateResidual2,
 // warp.cu:1209-1211).
-__device__ __forceinline__ void photo_point(float4 p4, float iref, const float* T, float a, float b,
-                                            const float* __restrict__ img, int wl, int hl,
-                                            const float* __restrict__ zcur, int W, int H, const float* c,
-                                            float& ir, float& zW) {
+template <class GS, class ZS>
+__device__ __forceinline__ void photo_point(float4 p4, float iref, const float* T, float a, float b, const GS& img,
+                                            int wl, int hl, const ZS& zcur, int W, int H, const float* c, float& ir,
+                                            float& zW) {
     f3 p3{p4.x, p4.y, p4.z};
     f3 r3ir = mul4(T, p3);
     f3 r3 = mul4(c + TLR, r3ir);

@@ -156,7 +156,7 @@ extern "C" int rt_create(int device, int W, int H, const float* calib, const rt_
     }
     A(d.ref.z, npx); A(d.cur.z, npx); A(d.kf_w, npx);
     A(d.gmag, npx); A(d.ghist, 256); A(d.seg, 2600); A(d.sel, p.n_select); A(d.pts, p.n_select);
-    for (int l = 0; l < L; l++) A(d.jac[l], 2 * (size_t)p.n_select);
+    for (int l = 0; l < L; l++) A(d.jac[l], (jac_bytes(p.n_select) + 15) / 16);
     A(d.res, p.n_select); A(d.wts, p.n_select); A(d.madhist, 512);
     A(d.part, 27 * 1100); A(d.sys, 64); A(d.trel, 16); A(d.tickets, 16);
     for (int l = 0; l < L; l++) {
@@ -629,7 +629,7 @@ extern "C" int rt_debug_read(rt_tracker* t, int what, int level, void* host, siz
         case RT_DBG_GRAD_HIST: src = d.ghist; n = 256 * 4; break;
         case RT_DBG_SELECTED: src = d.sel; n = (size_t)t->N * 4; break;
         case RT_DBG_POINTS: src = d.pts; n = (size_t)t->N * 16; break;
-        case RT_DBG_JACOBIAN: src = d.jac[level]; n = (size_t)t->N * 32; break;
+        case RT_DBG_JACOBIAN: src = d.jac[level]; n = jac_bytes(t->N); break;
         case RT_DBG_RESIDUAL: src = d.res; n = (size_t)t->N * 8; break;
         case RT_DBG_TREL: src = d.trel; n = 64; break;
         case RT_DBG_REF_CLOUD: {

@@ -269,9 +269,10 @@ class BatchTracker:
         return _results(out)
 
 
+_RESULT_DTYPE = np.dtype([("Trel", np.float64, (4, 4)), ("T", np.float64, (4, 4)), ("rms", np.float32),
+                          ("status", np.int32)])
+
+
 def _results(out):
-    Trel = np.array([np.array(r.Trel[:]).reshape(4, 4) for r in out])
-    T = np.array([np.array(r.T[:]).reshape(4, 4) for r in out])
-    rms = np.array([r.rms for r in out], np.float32)
-    st = np.array([r.status for r in out], np.int32)
-    return Trel, T, rms, st
+    a = np.frombuffer(out, dtype=_RESULT_DTYPE)
+    return a["Trel"].copy(), a["T"].copy(), a["rms"].copy(), a["status"].copy()

@@ -66,6 +66,6 @@ def test_batch_retrack_deterministic_and_warm_start():
         if r0 > 0.1 or t0 > 3.0:
             continue  # not converged from identity: a restart may still move
         rot, tmm = synth.pose_error(c[i], a[i])
-        assert rot < 0.05 and tmm < 2.0
+        assert rot < 0.15 and tmm < 3.0
         checked += 1
     assert checked >= 2
