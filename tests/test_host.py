@@ -129,3 +129,26 @@ def test_oracle_fp32_and_fp64_trackers_agree():
         out.append(o.poses()[0])
     rot, tr = synth.pose_error(out[0], out[1])
     assert rot < 1e-2 and tr < 0.1
+
+
+def test_cpp_phototracker_header_compiles_and_links(tmp_path):
+    """The PhotoTracker-compatible C++ class builds against the C ABI with a
+    plain host compiler and reports a rejected configuration the reference's
+    way (message on stderr, null tracker) without touching a device."""
+    import subprocess
+    src = tmp_path / "pt.cpp"
+    src.write_text(
+        '#include "rt_phototracker.hpp"\n'
+        "int main() {\n"
+        "    float cal[RT_CALIB_SIZE];\n"
+        "    if (rt_calib_pinhole(525, 525, 319.5f, 239.5f, 300, 7000, cal) != RT_OK) return 1;\n"
+        "    rt::PhotoTracker* t = rt::PhotoTracker::create(0, 2048, 1024, cal);\n"
+        "    return t == nullptr ? 0 : 2;\n"
+        "}\n")
+    exe = tmp_path / "pt"
+    libdir = os.path.join(ROOT, "recon_amd")
+    subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), str(src),
+                    "-L", libdir, "-lrecon_amd", f"-Wl,-rpath,{libdir}", "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0
+    assert "PhotoTracker::initializePhotoTracker" in r.stderr

@@ -534,3 +534,30 @@ def test_fusion_invariants():
     # 10 mm behind: averaged in with equal weight
     call("or_fuse_frame_d", kz2, kg2, kw2, z - 10.0, gray, W, H, np.eye(4).ravel(), c)
     assert np.allclose(kz2.reshape(H, W)[inner], (z - 5.0).reshape(H, W)[inner], atol=1e-9)
+
+
+def test_cloud_downsampling_masked_mean_on_a_plane():
+    """downSampleCloud6Kernel (d2z.cu:557-588): the masked 2x2 mean of points
+    on a plane stays on the plane with the plane's unit normal; masked taps do
+    not enter the mean; a block with no valid tap is masked out."""
+    W, H = 80, 60
+    c = calib(W, H)
+    n = np.array([0.1, 0.25, 1.0]); n /= np.linalg.norm(n)
+    d = -1500.0 * n[2]
+    z = _plane_depth(W, H, c, n, d)
+    pts = np.zeros(W * H * 6); m = np.zeros(W * H, np.int32)
+    call("or_ref_cloud_d", z, W, H, c, pts, m)
+    m2 = m.copy().reshape(H, W)
+    m2[10, 10] = 0                 # one masked tap in block (5, 5)
+    m2[20:22, 30:32] = 0           # a fully masked block (10, 15)
+    m2 = m2.ravel().copy()
+    Wd, Hd = W // 2, H // 2
+    dst = np.zeros(Wd * Hd * 6); md = np.zeros(Wd * Hd, np.int32)
+    call("or_downsample_cloud_d", pts, m2, Wd, Hd, dst, md)
+    D = dst.reshape(Hd, Wd, 6); M = md.reshape(Hd, Wd).astype(bool)
+    P = pts.reshape(H, W, 6)
+    assert not M[10, 15] and M[5, 5]
+    assert np.allclose(D[5, 5, :3], (P[10, 11, :3] + P[11, 10, :3] + P[11, 11, :3]) / 3, atol=1e-9)
+    inner = M.copy(); inner[0, :] = inner[-1, :] = False; inner[:, 0] = inner[:, -1] = False
+    assert np.allclose(D[inner][:, :3] @ n, d, atol=1e-6)
+    assert np.allclose(np.abs(D[inner][:, 3:] @ n), 1.0, atol=1e-9)
