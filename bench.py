@@ -1,12 +1,21 @@
 #!/usr/bin/env python3
-"""Benchmark: tracked frames/s at 640x480, batched persistent tracker vs the
-plain single-pair tracker on the same synthetic pairs.
+"""Benchmark: tracked frames/s at 640x480 of learned-proposal + batched
+Gauss-Newton tracking against the plain single-pair tracker, on the same
+synthetic pairs, with convergence rates and pose errors.
 
-A step is one pass of the hot path over one batch: keyframe preparation
-(conversion, pyramids, gradient histogram, point selection, Jacobians) and the
-coarse-to-fine photometric Gauss-Newton loop for every pair of the batch, from
-raw 8-bit gray / 16-bit depth frames resident in HBM to converged poses on the
-host. Prints one JSON line (see the repository's benchmark contract).
+Workload "learned" (default): 4096 pairs at 640x480 with large motions
+(rotation <= 10 deg, translation <= 150 mm). A step is one pass of the hot path
+over the batch, from raw 8-bit gray / 16-bit depth frames resident in HBM to
+converged poses on the host: keyframe preparation (pyramids, gradient
+histogram, point selection, Jacobians), the network's proposals (inputs,
+forward pass, initial poses) and the coarse-to-fine photometric Gauss-Newton
+loop of every pair in one persistent launch.
+Workload "small": the same without the network, motions <= 3 deg / 50 mm,
+identity start.
+
+Plain = the same kernels one pair at a time from identity. A pair has
+converged when its pose is within 0.1 deg / 1 mm of the judge: the pose Gauss-
+Newton converges to from the true motion. Prints one JSON line.
 """
 import argparse
 import ctypes
@@ -24,16 +33,20 @@ HBM_PEAK_GBS = 8000.0   # MI355X HBM3E
 # Algorithmic bytes per selected point per Gauss-Newton iteration: keyframe point
 # xyz 12 + reference gray 4 + Jacobian 24 + four bilinear gray taps 16 + depth tap 4.
 BYTES_PER_POINT_ITER = 60
+BF16_PEAK_TFLOPS = 2500.0   # MI355X dense bf16 MFMA
+# Convolution FLOPs of the proposal network per pair (2 * Hout * Wout * Cout * 9 * Cin summed
+# over the six layers at 80x60 input: 4-32-64-128-128-256-256)
+CNN_FLOPS_PER_PAIR = 2 * 9 * (40 * 30 * 32 * 4 + 20 * 15 * 64 * 32 + 10 * 8 * 128 * 64 + 5 * 4 * 128 * 128
+                              + 3 * 2 * 256 * 128 + 2 * 1 * 256 * 256)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def make_inputs(npairs, W, H, theta, tmax, seed0):
+def make_inputs(npairs, W, H, theta, tmax, seed0, nscenes=64):
     """Render npairs (reference, current) frames on the GPU into two device
-    buffers each for gray and depth; returns buffers, ground-truth Trel and
-    intrinsics."""
+    buffers each for gray and depth; returns buffers and ground-truth Trel."""
     from recon_amd import synth_lib
     from recon_amd.hip import DeviceBuffer, hip
     from recon_amd.synth import pinhole, sample_motion
@@ -44,7 +57,7 @@ def make_inputs(npairs, W, H, theta, tmax, seed0):
     d = DeviceBuffer(2 * npairs * npx * 2)
     gts = []
     dp = ctypes.POINTER(ctypes.c_double)
-    nscenes = min(npairs, 64)
+    nscenes = min(npairs, nscenes)
     scenes = [S.rs_scene_create(ctypes.c_uint64(seed0 + k)) for k in range(nscenes)]
     for i in range(npairs):
         sc = scenes[i % nscenes]
@@ -70,11 +83,11 @@ def pose_errors(Trel_est, Trel_gt):
     return e[:, 0], e[:, 1]
 
 
-def measured_traffic(W, H, B):
+def measured_traffic(W, H, B, workload):
     """HBM bytes per launch of the tracking kernel at this workload, from the
     committed rocprofv3 counter summary (FETCH_SIZE + WRITE_SIZE scaled by a
     copy kernel of known size); None where no summary exists for it."""
-    path = os.path.join(ROOT, "profiles", f"kb_track_{W}x{H}_{B}pairs.md")
+    path = os.path.join(ROOT, "profiles", f"kb_track_{W}x{H}_{B}pairs_{workload}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -142,12 +155,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=("learned", "small"), default="learned")
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--plain-pairs", type=int, default=64)
     ap.add_argument("--width", type=int, default=640)
     ap.add_argument("--height", type=int, default=480)
-    ap.add_argument("--theta", type=float, default=3.0, help="max rotation (deg) of a pair")
-    ap.add_argument("--tmax", type=float, default=50.0, help="max translation (mm) of a pair")
+    ap.add_argument("--theta", type=float, default=None, help="max rotation (deg) of a pair")
+    ap.add_argument("--tmax", type=float, default=None, help="max translation (mm) of a pair")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "recon_amd", "data", "pose_cnn.npz"))
+    ap.add_argument("--oracle-pairs", type=int, default=8, help="pairs checked against the FP64 oracle (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall time of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--seed", type=int, default=4)
@@ -166,41 +182,61 @@ def main():
               "(pairs are independent: shard them across GPUs and gather poses)", file=sys.stderr)
         sys.exit(2)
 
-    from recon_amd import PHOTOMETRIC, RtPair
+    from recon_amd import PHOTOMETRIC
     from recon_amd.hip import Event, stream_sync
     from recon_amd.synth import calib_block
     from recon_amd.tracker import BatchTracker, PhotoTracker
 
+    learned = args.workload == "learned"
+    theta = args.theta if args.theta is not None else (10.0 if learned else 3.0)
+    tmax = args.tmax if args.tmax is not None else (150.0 if learned else 50.0)
     W, H, B = args.width, args.height, args.pairs
     iters = (1, 3, 10, 5)
     N = 8192
     L = 4 if W == 640 else 3
     t0 = time.time()
-    g, d, gt = make_inputs(B, W, H, args.theta, args.tmax, args.seed)
-    log(f"inputs: {B} pairs {W}x{H} rendered in {time.time() - t0:.1f}s")
+    g, d, gt = make_inputs(B, W, H, theta, tmax, args.seed, nscenes=B if learned else 64)
+    log(f"inputs: {B} pairs {W}x{H} (<= {theta} deg, <= {tmax} mm) rendered in {time.time() - t0:.1f}s")
     npx = W * H
     cal = calib_block(W, H)
+    pairs = [dict(ref_gray=g.ptr + 2 * i * npx, ref_depth=d.ptr + 2 * i * npx * 2, cur_gray=g.ptr + (2 * i + 1) * npx,
+                  cur_depth=d.ptr + (2 * i + 1) * npx * 2, width=W, height=H, calib=cal) for i in range(B)]
+    bt = BatchTracker(B, W, H, mode=PHOTOMETRIC, n_select=N, iters=iters)
+    stream = bt.L.rt_batch_stream(bt.h)
+    prepared = bt.prepare(pairs)
+    net = None
+    if learned:
+        from recon_amd.cnn import PoseNet
+        net = PoseNet.load(args.weights, max_batch=B)
+
+    def step():
+        if learned:
+            bt.prepare_only(prepared)
+            bt.propose(net)
+            return bt.retrack(None, B)
+        return bt.track(prepared)
 
     if args.track_only:
-        bt = BatchTracker(B, W, H, mode=PHOTOMETRIC, n_select=N, iters=iters)
-        pairs = [dict(ref_gray=g.ptr + 2 * i * npx, ref_depth=d.ptr + 2 * i * npx * 2,
-                      cur_gray=g.ptr + (2 * i + 1) * npx, cur_depth=d.ptr + (2 * i + 1) * npx * 2,
-                      width=W, height=H, calib=cal) for i in range(B)]
-        bt.track(bt.prepare(pairs))
+        step()
         for _ in range(args.steps):
             bt.L.rt_batch_retrack(bt.h, None, None)
-        stream_sync(bt.L.rt_batch_stream(bt.h))
+        stream_sync(stream)
         log(f"track-only: {args.steps} tracking launches over {B} pairs done")
         return
 
-    # ---- plain single-pair tracker (same kernels, one pair at a time)
+    # ---- judge: Gauss-Newton from the true motion (untimed)
+    judge, _, _, judge_st = bt.track(prepared, init=list(gt))
+    rj, tj = pose_errors(judge, gt)
+    log(f"judge (GN from truth): median {np.median(rj):.4f} deg {np.median(tj):.3f} mm vs truth")
+
+    # ---- plain single-pair tracker (same kernels, one pair at a time, identity start)
     P = min(args.plain_pairs, B)
     tr = PhotoTracker(W, H, cal, mode=PHOTOMETRIC, n_select=N, iters=iters)
     plain_T = []
 
     def plain_pair(i):
-        tr.set_reference_points(g.ptr + 2 * i * npx, d.ptr + 2 * i * npx * 2, mode=PHOTOMETRIC, force=True, on_device=True)
-        tr.set_current_image(g.ptr + (2 * i + 1) * npx, d.ptr + (2 * i + 1) * npx * 2, on_device=True)
+        tr.set_reference_points(pairs[i]["ref_gray"], pairs[i]["ref_depth"], mode=PHOTOMETRIC, force=True, on_device=True)
+        tr.set_current_image(pairs[i]["cur_gray"], pairs[i]["cur_depth"], on_device=True)
         tr.track()
         return tr.trel().astype(np.float64)
 
@@ -215,24 +251,22 @@ def main():
     plain_fps = P / plain_s
     log(f"plain single-pair: {P} pairs in {plain_s * 1e3:.1f} ms -> {plain_fps:.1f} frames/s")
 
-    # ---- batched persistent tracker
-    bt = BatchTracker(B, W, H, mode=PHOTOMETRIC, n_select=N, iters=iters)
-    pairs = [dict(ref_gray=g.ptr + 2 * i * npx, ref_depth=d.ptr + 2 * i * npx * 2, cur_gray=g.ptr + (2 * i + 1) * npx,
-                  cur_depth=d.ptr + (2 * i + 1) * npx * 2, width=W, height=H, calib=cal) for i in range(B)]
-    prepared = bt.prepare(pairs)
+    # ---- batched identity start (untimed, for the convergence comparison)
+    ident, _, _, _ = bt.retrack(np.tile(np.eye(4).ravel(), B), B)
+
+    # ---- the timed step
     for _ in range(max(1, args.warmup)):
-        Trel_b, _, rms, st = bt.track(prepared)
-    stream = bt.L.rt_batch_stream(bt.h)
+        Trel_b, _, rms, st = step()
     stream_sync(stream)
     t2 = time.perf_counter()
     for _ in range(args.steps):
-        Trel_b, _, rms, st = bt.track(prepared)
+        Trel_b, _, rms, st = step()
     stream_sync(stream)
     step_s = (time.perf_counter() - t2) / args.steps
     value = B / step_s
 
-    # ---- roofline of the dominant kernel (the persistent tracking launch), HIP events on its stream
-    ev0, ev1 = Event(), Event()
+    # ---- dominant kernel (the persistent tracking launch), HIP events on its stream
+    ev0, ev1, ev2 = Event(), Event(), Event()
     reps = 3
     ev0.record(stream)
     for _ in range(reps):
@@ -242,15 +276,41 @@ def main():
     k_ms = ev0.elapsed_ms(ev1) / reps
     alg_bytes = B * sum(iters[:L]) * N * BYTES_PER_POINT_ITER
     achieved = alg_bytes / (k_ms * 1e-3) / 1e9
+    cnn = None
+    if learned:
+        ev0.record(stream)
+        for _ in range(reps):
+            bt.propose(net)
+        ev2.record(stream)
+        ev2.synchronize()
+        c_ms = ev0.elapsed_ms(ev2) / reps
+        flops = B * CNN_FLOPS_PER_PAIR
+        cnn = {"bound": "mfma", "achieved": round(flops / (c_ms * 1e-3) / 1e12, 2), "peak": BF16_PEAK_TFLOPS,
+               "unit": "TFLOP/s", "frac": round(flops / (c_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 5),
+               "ms": round(c_ms, 3), "flops_per_pair": CNN_FLOPS_PER_PAIR,
+               "note": "proposal launches (inputs + six convolutions + head + poses) timed together"}
 
-    # ---- errors
-    rb, tb = pose_errors(Trel_b, gt)
-    rp, tp = pose_errors(plain_T, gt[:P])
-    ra, ta = pose_errors(Trel_b[:P], plain_T)
-    log(f"batched: {B} pairs, {step_s * 1e3:.2f} ms/step -> {value:.0f} frames/s ({value / plain_fps:.1f}x plain); "
-        f"kernel {k_ms:.2f} ms, {achieved:.0f} GB/s algorithmic")
-    log(f"pose error vs truth: batched median {np.median(rb):.4f} deg {np.median(tb):.3f} mm, "
-        f"plain median {np.median(rp):.4f} deg {np.median(tp):.3f} mm; batched vs plain max {ra.max():.2e} deg {ta.max():.2e} mm")
+    # ---- convergence and errors
+    def conv(T, ref):
+        r, t = pose_errors(T, ref)
+        ok = (r <= 0.1) & (t <= 1.0)
+        return ok, r, t
+    ok_b, rb, tb = conv(Trel_b, judge)
+    ok_i, _, _ = conv(ident, judge)
+    ok_p, rp, tp = conv(np.array(plain_T), judge[:P])
+    ra, ta = pose_errors(ident[:P], plain_T)
+    rtb, ttb = pose_errors(Trel_b, gt)
+    log(f"{'learned+batched' if learned else 'batched'}: {B} pairs, {step_s * 1e3:.2f} ms/step -> {value:.0f} frames/s "
+        f"({value / plain_fps:.1f}x plain); kb_track {k_ms:.2f} ms, {achieved:.0f} GB/s algorithmic")
+    log(f"converged (<=0.1 deg / 1 mm of the judge): {'learned+batched' if learned else 'batched'} {ok_b.mean() * 100:.1f}%, "
+        f"batched from identity {ok_i.mean() * 100:.1f}%, plain from identity {ok_p.mean() * 100:.1f}% (of {P}); "
+        f"batched vs plain from identity max {ra.max():.2e} deg {ta.max():.2e} mm")
+
+    # ---- FP64 oracle from the true motion on a sample: error of the converged results
+    fp64 = None
+    if args.oracle_pairs > 0:
+        fp64 = fp64_check(g, d, gt, Trel_b, ok_b, min(args.oracle_pairs, B), W, H, L, N, iters, cal, args.cpu_threads)
+        log(f"vs FP64 oracle: {fp64}")
 
     # ---- CPU baseline: the oracle's FP32 transcription on the host cores
     cpu = None
@@ -258,6 +318,7 @@ def main():
         cpu = cpu_baseline(g, d, min(B, 64), W, H, L, N, iters, cal, args.cpu_seconds, args.cpu_threads)
         log(f"cpu baseline: {cpu['value']} frames/s ({cpu['sample']})")
 
+    name = "learned proposal + batched Gauss-Newton" if learned else "batched Gauss-Newton from identity"
     out = {
         "metric": "tracked frames/s at 640x480 (learned+batched vs plain GN) at matched pose error",
         "value": round(value, 1),
@@ -269,23 +330,62 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64",
+        "dtype": "f32+f64" + ("+bf16" if learned else ""),
         "data": "synthetic",
-        "config": {"workload": f"batched photometric tracking of {B} frame pairs at {W}x{H} (rotation<={args.theta} deg, "
-                               f"translation<={args.tmax} mm), {N} points, {L} levels, iterations {list(iters[:L])}, identity start",
-                   "pairs": B, "width": W, "height": H, "points": N, "levels": L},
+        "config": {"workload": f"{name}: {B} frame pairs at {W}x{H}, rotation <= {theta} deg, translation <= {tmax} mm, "
+                               f"{N} points, {L} levels, iterations {list(iters[:L])}",
+                   "pairs": B, "width": W, "height": H, "points": N, "levels": L, "theta_deg": theta, "tmax_mm": tmax},
         "plain_fps": round(plain_fps, 1),
         "speedup_vs_plain": round(value / plain_fps, 2),
-        "pose_error": {"batched_median_deg": float(np.median(rb)), "batched_median_mm": float(np.median(tb)),
-                       "plain_median_deg": float(np.median(rp)), "plain_median_mm": float(np.median(tp)),
-                       "batched_vs_plain_max_deg": float(ra.max()), "batched_vs_plain_max_mm": float(ta.max())},
+        "convergence": {"criterion": "within 0.1 deg and 1 mm of Gauss-Newton started at the true motion",
+                        "this": round(float(ok_b.mean()), 4), "batched_from_identity": round(float(ok_i.mean()), 4),
+                        "plain_from_identity": round(float(ok_p.mean()), 4), "plain_sample": P},
+        "pose_error": {"converged_median_deg_vs_truth": float(np.median(rtb[ok_b])) if ok_b.any() else None,
+                       "converged_median_mm_vs_truth": float(np.median(ttb[ok_b])) if ok_b.any() else None,
+                       "judge_median_deg_vs_truth": float(np.median(rj)), "judge_median_mm_vs_truth": float(np.median(tj)),
+                       "batched_vs_plain_identity_max_deg": float(ra.max()),
+                       "batched_vs_plain_identity_max_mm": float(ta.max()), "vs_fp64_oracle": fp64},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(W, H, B),
-                     "algorithmic_bytes": alg_bytes,
-                     "kernel": "kb_track (persistent Gauss-Newton)", "kernel_ms": round(k_ms, 3)},
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(W, H, B, args.workload),
+                     "algorithmic_bytes": alg_bytes, "kernel": "kb_track (persistent Gauss-Newton)",
+                     "kernel_ms": round(k_ms, 3)},
+        "cnn_roofline": cnn,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def fp64_check(g, d, gt, Trel, ok, n, W, H, L, N, iters, cal, threads):
+    """The FP64 oracle tracker started at the true motion for the first n
+    pairs; errors of the converged results of those pairs against it."""
+    import threading
+
+    import oracle
+    from recon_amd.synth import pose_error
+    npx = W * H
+    gh = g.to_array(np.uint8, 2 * n * npx).reshape(2 * n, H, W)
+    dh = d.to_array(np.uint16, 2 * n * npx).reshape(2 * n, H, W)
+    res = [None] * n
+
+    def work(i):
+        o = oracle.Tracker(W, H, cal, mode=oracle.PHOTOMETRIC, levels=L, iters=iters, nselect=N, dtype=np.float64)
+        o.set_reference(gh[2 * i], dh[2 * i])
+        o.set_current(gh[2 * i + 1], dh[2 * i + 1])
+        o.prepare()
+        o.set_trel(gt[i])
+        o.track()
+        res[i] = o.poses()[0]
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(n)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    e = np.array([pose_error(Trel[i], res[i]) for i in range(n)])
+    sel = ok[:n]
+    return {"pairs": n, "converged": int(sel.sum()),
+            "max_deg": float(e[sel, 0].max()) if sel.any() else None,
+            "max_mm": float(e[sel, 1].max()) if sel.any() else None}
 
 
 if __name__ == "__main__":

@@ -193,9 +193,11 @@ void rt_batch_destroy(rt_batch* b);
 /* Prepare keyframes and current pyramids for n pairs and run the tracking
  * launch; results are copied to out (host) when the call returns. */
 int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_result* out);
-/* Re-run only the tracking launch on the pairs of the last rt_batch_track
- * (inputs already prepared in device memory), from the given initial poses
- * (nullable: identity). For timing and warm starts. */
+/* Re-run only the tracking launch on the pairs of the last rt_batch_track /
+ * rt_batch_prepare (inputs already prepared in device memory), from the given
+ * initial poses (n x 16 doubles, host), or, with init_T NULL, from the initial
+ * poses already on the device (those of the last call, or the proposals of
+ * rt_batch_propose). out NULL: enqueue without waiting. */
 int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_result* out);
 void* rt_batch_stream(rt_batch* b);
 

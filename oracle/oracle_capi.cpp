@@ -1,6 +1,7 @@
 // C entry points of the CPU oracle (TEST INFRASTRUCTURE ONLY, see oracle.hpp),
 // loaded by the tests through ctypes. Suffix _f: FP32 literal transcription,
 // _d: FP64 oracle.
+#include "cnn.hpp"
 #include "tracker.hpp"
 
 using namespace oracle;
@@ -172,5 +173,16 @@ static Params make_params(const int* p, float lambda) {
         std::memcpy(w, t->kf_fw.data(), sizeof(R) * t->kf_fw.size());                                \
     }
 OR_BOTH(DEF_TRACKER)
+
+// learned pose proposal (cnn.hpp)
+size_t or_cnn_num_params(const int* ch, int H0, int W0) { return cnn::Net(ch, H0, W0).nparams; }
+double or_cnn_forward_backward(const int* ch, int H0, int W0, int B, const float* params, const float* x,
+                               const float* y, const float* wn, int round, double* out, double* grad) {
+    cnn::Net net(ch, H0, W0);
+    return cnn::forward_backward(net, B, params, x, y, wn, round != 0, out, grad);
+}
+void or_bf16_round(const float* x, int n, float* out) {
+    for (int i = 0; i < n; i++) out[i] = cnn::bf16_round(x[i]);
+}
 
 }  // extern "C"

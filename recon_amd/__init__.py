@@ -37,7 +37,12 @@ class RtPairResult(ctypes.Structure):
     _fields_ = [("Trel", c_double * 16), ("T", c_double * 16), ("rms", c_float), ("status", c_int)]
 
 
-# name -> (restype, argtypes); the exported symbols of include/rt_tracker.h
+class RtCnnConfig(ctypes.Structure):
+    _fields_ = [("in_width", c_int), ("in_height", c_int), ("channels", c_int * 7), ("rot_scale_deg", c_float),
+                ("trans_scale_mm", c_float)]
+
+
+# name -> (restype, argtypes); the exported symbols of include/rt_tracker.h and include/rt_cnn.h
 SIGNATURES = {
     "rt_default_params": (None, [P(RtParams)]),
     "rt_calib_pinhole": (c_int, [c_float, c_float, c_float, c_float, c_float, c_float, P(c_float)]),
@@ -81,6 +86,22 @@ SIGNATURES = {
     "rt_debug_prepare": (c_int, [c_void_p]),
     "rt_debug_linearize": (c_int, [c_void_p, c_int, c_int]),
     "rt_last_error": (ctypes.c_char_p, []),
+    "rt_cnn_default_config": (None, [P(RtCnnConfig)]),
+    "rt_cnn_create": (c_int, [c_int, P(RtCnnConfig), c_int, P(c_void_p)]),
+    "rt_cnn_destroy": (None, [c_void_p]),
+    "rt_cnn_num_params": (c_size_t, [c_void_p]),
+    "rt_cnn_set_params": (c_int, [c_void_p, P(c_float)]),
+    "rt_cnn_get_params": (c_int, [c_void_p, P(c_float)]),
+    "rt_cnn_stream": (c_void_p, [c_void_p]),
+    "rt_cnn_make_inputs": (c_int, [c_void_p, c_int, P(RtPair), c_void_p]),
+    "rt_cnn_forward": (c_int, [c_void_p, c_int, c_void_p, c_void_p]),
+    "rt_cnn_loss_grad": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, P(c_float), P(c_float), P(c_float)]),
+    "rt_cnn_train_step": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_float, P(c_float)]),
+    "rt_cnn_augment": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, P(c_int), c_void_p, c_void_p]),
+    "rt_cnn_pose_to_target": (c_int, [P(RtCnnConfig), c_int, P(c_double), P(c_float)]),
+    "rt_cnn_target_to_pose": (c_int, [P(RtCnnConfig), c_int, P(c_float), P(c_double)]),
+    "rt_batch_prepare": (c_int, [c_void_p, c_int, P(RtPair)]),
+    "rt_batch_propose": (c_int, [c_void_p, c_void_p]),
 }
 
 GEOMETRIC, PHOTOMETRIC, BIOBJECTIVE = 0, 1, 2

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["rt_kernels.hip", "rt_tracker.hip", "rt_batch.hip", "synth.hip"]
+HIP_SOURCES = ["rt_kernels.hip", "rt_tracker.hip", "rt_batch.hip", "cnn.hip", "synth.hip"]
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
     # every FP32 value is the rounded result of the written operation sequence
@@ -31,10 +31,22 @@ def _run(cmd):
 
 
 def build_hip():
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES if os.path.exists(os.path.join(CSRC, s))]
-    out = os.path.join(HERE, "librecon_amd.so")
+    """Compile each source to an object in parallel, then link the library."""
+    from concurrent.futures import ThreadPoolExecutor
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    _run([hipcc] + HIPCC_FLAGS + ["-I", os.path.join(ROOT, "include"), "-o", out] + srcs)
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    compile_flags = [f for f in HIPCC_FLAGS if f != "-shared"]
+
+    def obj(src):
+        o = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        _run([hipcc] + compile_flags + ["-I", os.path.join(ROOT, "include"), "-c", os.path.join(CSRC, src), "-o", o])
+        return o
+
+    with ThreadPoolExecutor(max_workers=min(len(HIP_SOURCES), os.cpu_count() or 1)) as ex:
+        objs = list(ex.map(obj, HIP_SOURCES))
+    out = os.path.join(HERE, "librecon_amd.so")
+    _run([hipcc, "-shared", "-fPIC", "--offload-arch=gfx950", "-o", out] + objs)
     return out
 
 

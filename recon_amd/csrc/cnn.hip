@@ -1,0 +1,858 @@
+// Pose-proposal network: forward and backward kernels on the CDNA4 matrix
+// cores (bfloat16 operands, FP32 accumulation), Adam, and the C ABI of
+// include/rt_cnn.h.
+//
+// Every convolution is an implicit GEMM over NHWC bfloat16 tensors, built from
+// v_mfma_f32_16x16x16_bf16 tiles (64-lane wave, A: 16 rows x 4 k per lane
+// group, B: 4 k x 16 columns, D: 4 rows x 16 columns per lane group):
+//   forward      rows = output pixels (n, yo, xo), columns = Cout, k = (tap, Cin)
+//   input grad   rows = input pixels (n, yi, xi),  columns = Cin,  k = (tap, Cout)
+//   weight grad  rows = Cout, columns = (tap, Cin), k = output pixels, split
+//                over workgroups and summed in a fixed order (deterministic)
+// The head (global pooling, the 6-output dense layer, loss) is FP32.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/rt_cnn.h"
+#include "cnn.hpp"
+#include "rt_host.hpp"
+
+namespace rt {
+
+typedef uint16_t bf16;
+typedef short s4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf2f(bf16 v) { return __uint_as_float((uint32_t)v << 16); }
+// round to nearest even; NaN stays quiet NaN
+__device__ __forceinline__ bf16 f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u) {
+        if (u & 0x7fffffu) u |= 0x400000u;
+        return (bf16)(u >> 16);
+    }
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16)(u >> 16);
+}
+__device__ __forceinline__ bool bf_pos(bf16 v) { return !(v & 0x8000u) && v != 0; }
+
+struct ConvGeom {
+    int B, Hi, Wi, Ci, ci_shift, Ho, Wo, Co, co_shift, K, Kp;
+};
+
+__device__ __forceinline__ f4 mfma(s4 a, s4 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0); }
+
+// ---------------------------------------------------------------- forward
+// Block: 4 waves, 64 output pixels x 64 output channels; wave: 32 x 32.
+__global__ __launch_bounds__(256) void k_conv_fwd(ConvGeom g, const bf16* __restrict__ X, const bf16* __restrict__ Wp,
+                                                  const float* __restrict__ bias, bf16* __restrict__ Y) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int M = g.B * g.Ho * g.Wo;
+    const int row0 = blockIdx.x * 64 + (w & 1) * 32;
+    const int col0 = blockIdx.y * 64 + (w >> 1) * 32;
+    if (col0 >= g.Co || row0 >= M) return;
+    const int HW = g.Ho * g.Wo;
+    int nn[2], yo[2], xo[2];
+    bool mv[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        int m = row0 + i * 16 + r16;
+        mv[i] = m < M;
+        int mm = mv[i] ? m : 0;
+        nn[i] = mm / HW;
+        int r = mm - nn[i] * HW;
+        yo[i] = r / g.Wo;
+        xo[i] = r - yo[i] * g.Wo;
+    }
+    f4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    const s4 zero = {0, 0, 0, 0};
+    for (int k0 = kq * 4; k0 < g.Kp; k0 += 16) {
+        const int tap = k0 >> g.ci_shift, ci = k0 & (g.Ci - 1);
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+        s4 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            int yi = 2 * yo[i] - 1 + ky, xi = 2 * xo[i] - 1 + kx;
+            bool ok = mv[i] && tap < 9 && yi >= 0 && yi < g.Hi && xi >= 0 && xi < g.Wi;
+            a[i] = ok ? *(const s4*)(X + (((size_t)nn[i] * g.Hi + yi) * g.Wi + xi) * g.Ci + ci) : zero;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int c = col0 + j * 16 + r16;
+            b[j] = c < g.Co ? *(const s4*)(Wp + (size_t)c * g.Kp + k0) : zero;
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        int col = col0 + j * 16 + r16;
+        if (col >= g.Co) continue;
+        float bv = bias[col];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                int m = row0 + i * 16 + kq * 4 + e;
+                if (m < M) {
+                    float v = acc[i][j][e] + bv;
+                    Y[(size_t)m * g.Co + col] = f2bf(v > 0.0f ? v : 0.0f);
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------- input gradient
+// dX = conv^T(dZ, W), masked by the ReLU of the layer input (X > 0), stored as
+// the next (earlier) layer's dZ. WT: [Cin][9][Cout].
+__global__ __launch_bounds__(256) void k_conv_bwd_data(ConvGeom g, const bf16* __restrict__ dZ,
+                                                       const bf16* __restrict__ WT, const bf16* __restrict__ X,
+                                                       bf16* __restrict__ dX) {
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int M = g.B * g.Hi * g.Wi;
+    const int row0 = blockIdx.x * 64 + (w & 1) * 32;
+    const int col0 = blockIdx.y * 64 + (w >> 1) * 32;
+    if (col0 >= g.Ci || row0 >= M) return;
+    const int HW = g.Hi * g.Wi;
+    int nn[2], yi[2], xi[2];
+    bool mv[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        int m = row0 + i * 16 + r16;
+        mv[i] = m < M;
+        int mm = mv[i] ? m : 0;
+        nn[i] = mm / HW;
+        int r = mm - nn[i] * HW;
+        yi[i] = r / g.Wi;
+        xi[i] = r - yi[i] * g.Wi;
+    }
+    f4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    const s4 zero = {0, 0, 0, 0};
+    const int Kd = 9 * g.Co;
+    for (int k0 = kq * 4; k0 < Kd; k0 += 16) {
+        const int tap = k0 >> g.co_shift, co = k0 & (g.Co - 1);
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+        s4 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++) {
+            int ty = yi[i] + 1 - ky, tx = xi[i] + 1 - kx;
+            int yo = ty >> 1, xo = tx >> 1;
+            bool ok = mv[i] && ty >= 0 && tx >= 0 && !(ty & 1) && !(tx & 1) && yo < g.Ho && xo < g.Wo;
+            a[i] = ok ? *(const s4*)(dZ + (((size_t)nn[i] * g.Ho + yo) * g.Wo + xo) * g.Co + co) : zero;
+        }
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int c = col0 + j * 16 + r16;
+            b[j] = c < g.Ci ? *(const s4*)(WT + ((size_t)c * 9 + tap) * g.Co + co) : zero;
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        int col = col0 + j * 16 + r16;
+        if (col >= g.Ci) continue;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                int m = row0 + i * 16 + kq * 4 + e;
+                if (m < M) {
+                    size_t o = (size_t)m * g.Ci + col;
+                    dX[o] = f2bf(bf_pos(X[o]) ? acc[i][j][e] : 0.0f);
+                }
+            }
+    }
+}
+
+// ---------------------------------------------------------------- weight gradient
+// ws[s][co][k] = sum over output pixels m of chunk s of dZ[m][co] * im2col(X)[m][k].
+// One wave per block: 32 output channels x 32 k.
+__global__ __launch_bounds__(64) void k_conv_bwd_w(ConvGeom g, const bf16* __restrict__ dZ, const bf16* __restrict__ X,
+                                                   float* __restrict__ ws, int chunk) {
+    const int lane = threadIdx.x;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int M = g.B * g.Ho * g.Wo, HW = g.Ho * g.Wo;
+    const int co0 = blockIdx.x * 32, kc0 = blockIdx.y * 32, s = blockIdx.z;
+    const int mbeg = s * chunk, mend = min(M, mbeg + chunk);
+    int tapj[2], cij[2];
+    bool kv[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+        int k = kc0 + j * 16 + r16;
+        tapj[j] = k >> g.ci_shift;
+        cij[j] = k & (g.Ci - 1);
+        kv[j] = k < g.K;
+    }
+    f4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int m0 = mbeg; m0 < mend; m0 += 16) {
+        s4 a[2], b[2];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            int m = m0 + kq * 4 + e;
+            bool mok = m < mend;
+            int mm = mok ? m : mbeg;
+            int n = mm / HW, r = mm - (mm / HW) * HW;
+            int yo = r / g.Wo, xo = r - (r / g.Wo) * g.Wo;
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                int co = co0 + i * 16 + r16;
+                a[i][e] = (mok && co < g.Co) ? (short)dZ[(size_t)mm * g.Co + co] : (short)0;
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                int tap = tapj[j];
+                int ky = tap / 3, kx = tap - 3 * (tap / 3);
+                int yi = 2 * yo - 1 + ky, xi = 2 * xo - 1 + kx;
+                bool ok = mok && kv[j] && yi >= 0 && yi < g.Hi && xi >= 0 && xi < g.Wi;
+                b[j][e] = ok ? (short)X[(((size_t)n * g.Hi + yi) * g.Wi + xi) * g.Ci + cij[j]] : (short)0;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            int k = kc0 + j * 16 + r16;
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                int co = co0 + i * 16 + kq * 4 + e;
+                if (co < g.Co && k < g.Kp) ws[((size_t)s * g.Co + co) * g.Kp + k] = acc[i][j][e];
+            }
+        }
+}
+
+// grad_w[co][k] = sum_s ws[s][co][k] (k < K), in order of s
+__global__ void k_reduce_w(const float* __restrict__ ws, int S, int Co, int Kp, int K, float* __restrict__ gw) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Co * K) return;
+    int co = idx / K, k = idx - (idx / K) * K;
+    float s = 0.0f;
+    for (int q = 0; q < S; q++) s += ws[((size_t)q * Co + co) * Kp + k];
+    gw[idx] = s;
+}
+
+// per-chunk column sums of dZ (bias gradient), then their ordered sum
+__global__ void k_colsum(const bf16* __restrict__ dZ, int M, int C, int chunk, float* __restrict__ ws) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = blockIdx.y;
+    if (c >= C) return;
+    int mbeg = s * chunk, mend = min(M, mbeg + chunk);
+    float acc = 0.0f;
+    for (int m = mbeg; m < mend; m++) acc += bf2f(dZ[(size_t)m * C + c]);
+    ws[(size_t)s * C + c] = acc;
+}
+__global__ void k_reduce_rows(const float* __restrict__ ws, int S, int width, float* __restrict__ out) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= width) return;
+    float s = 0.0f;
+    for (int q = 0; q < S; q++) s += ws[(size_t)q * width + k];
+    out[k] = s;
+}
+
+// ---------------------------------------------------------------- head
+// One block per sample: mean pool, dense 6 outputs; with targets also the loss,
+// the dense-layer gradient contributions and dZ of the last convolution.
+// 1 / max(1, sum of the sample weights), in order
+__global__ void k_weight_norm(const float* __restrict__ wn, int n, float* __restrict__ inv_norm) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    float s = 0.0f;
+    for (int i = 0; i < n; i++) s += wn[i];
+    *inv_norm = 1.0f / fmaxf(1.0f, s);
+}
+
+__global__ __launch_bounds__(256) void k_head(int P6, int C6, const bf16* __restrict__ A6, const float* __restrict__ Wfc,
+                                              const float* __restrict__ bfc, const float* __restrict__ y,
+                                              const float* __restrict__ wn, const float* __restrict__ inv_norm_p,
+                                              float* __restrict__ out,
+                                              bf16* __restrict__ dZ6, float* __restrict__ ws_fc, float* __restrict__ ws_loss) {
+    extern __shared__ float sh[];  // h[C6], dh[C6]
+    __shared__ float red[4][6];
+    __shared__ float o6[6], d6[6];
+    const int n = blockIdx.x, t = threadIdx.x, lane = t & 63, wv = t >> 6;
+    const float inv_norm = y ? *inv_norm_p : 0.0f;
+    float* h = sh;
+    for (int c = t; c < C6; c += 256) {
+        float s = 0.0f;
+        for (int p = 0; p < P6; p++) s += bf2f(A6[((size_t)n * P6 + p) * C6 + c]);
+        h[c] = s / (float)P6;
+    }
+    __syncthreads();
+    for (int j = 0; j < 6; j++) {
+        float s = 0.0f;
+        for (int c = t; c < C6; c += 256) s += Wfc[j * C6 + c] * h[c];
+        for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+        if (lane == 0) red[wv][j] = s;
+    }
+    __syncthreads();
+    if (t < 6) {
+        float v = bfc[t] + ((red[0][t] + red[1][t]) + (red[2][t] + red[3][t]));
+        o6[t] = v;
+        out[n * 6 + t] = v;
+        if (y) d6[t] = 2.0f * wn[n] * (v - y[n * 6 + t]) * inv_norm;
+    }
+    __syncthreads();
+    if (!y) return;
+    if (t == 0) {
+        float l = 0.0f;
+        for (int j = 0; j < 6; j++) { float e = o6[j] - y[n * 6 + j]; l += e * e; }
+        ws_loss[n] = wn[n] * l * inv_norm;
+    }
+    const int width = 6 * C6 + 6;
+    float* dh = sh + C6;
+    for (int c = t; c < C6; c += 256) {
+        float s = 0.0f;
+        for (int j = 0; j < 6; j++) {
+            ws_fc[(size_t)n * width + j * C6 + c] = d6[j] * h[c];
+            s += Wfc[j * C6 + c] * d6[j];
+        }
+        dh[c] = s;
+    }
+    if (t < 6) ws_fc[(size_t)n * width + 6 * C6 + t] = d6[t];
+    __syncthreads();
+    for (int q = t; q < P6 * C6; q += 256) {
+        int c = q % C6;
+        size_t o = (size_t)n * P6 * C6 + q;
+        dZ6[o] = f2bf(bf_pos(A6[o]) ? dh[c] / (float)P6 : 0.0f);
+    }
+}
+
+// ---------------------------------------------------------------- parameters
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       size_t n, float lr, float b1, float b2, float c1, float c2, float eps) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    float gi = g[i];
+    float mi = b1 * m[i] + (1.0f - b1) * gi;
+    float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+    m[i] = mi; v[i] = vi;
+    p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+// bfloat16 copies of one layer's weights: Wp[co][Kp] (zero padded) and WT[ci][tap][co]
+__global__ void k_pack(const float* __restrict__ W, int Co, int Ci, int Kp, bf16* __restrict__ Wp, bf16* __restrict__ WT) {
+    int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= Co * Kp) return;
+    int co = idx / Kp, k = idx - (idx / Kp) * Kp;
+    int K = 9 * Ci;
+    bf16 v = k < K ? f2bf(W[(size_t)co * K + k]) : (bf16)0;
+    Wp[idx] = v;
+    if (WT && k < K) {
+        int tap = k / Ci, ci = k - (k / Ci) * Ci;
+        WT[((size_t)ci * 9 + tap) * Co + co] = v;
+    }
+}
+
+// ---------------------------------------------------------------- inputs and proposals
+__global__ __launch_bounds__(256) void k_inputs(const FrameSrc* __restrict__ src, int IW, int IH, bf16* __restrict__ x) {
+    const int n = blockIdx.y;
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    if (cell >= IW * IH) return;
+    const FrameSrc s = src[n];
+    const int bw = s.W / IW, bh = s.H / IH;
+    const int cx = cell % IW, cy = cell / IW;
+    float feat[4];
+    for (int f = 0; f < 2; f++) {
+        unsigned gs = 0;
+        float ids = 0.0f;
+        int nv = 0;
+        for (int yy = 0; yy < bh; yy++) {
+            size_t row = (size_t)(cy * bh + yy) * s.W + (size_t)cx * bw;
+            for (int xx = 0; xx < bw; xx++) {
+                gs += s.g8[f][row + xx];
+                unsigned d = s.d16[f][row + xx];
+                if (d) { ids += 1000.0f / (float)d; nv++; }
+            }
+        }
+        feat[2 * f] = (float)gs / (255.0f * (float)(bw * bh)) - 0.5f;
+        feat[2 * f + 1] = nv ? ids / (float)nv - 1.0f : -1.0f;
+    }
+    bf16* o = x + ((size_t)n * IW * IH + cell) * 4;
+    for (int c = 0; c < 4; c++) o[c] = f2bf(feat[c]);
+}
+
+// Training-set symmetries of a pinhole camera with a centred principal point.
+// flags bit 0: mirror columns (camera x -> -x), bit 1: mirror rows (y -> -y),
+// bit 2: swap reference and current frame (target = that of the inverse pose,
+// y_inv). A mirror S maps Trel to S Trel S: rotation vector -S w, translation S t.
+__global__ __launch_bounds__(256) void k_augment(const bf16* __restrict__ x, const float* __restrict__ y,
+                                                 const float* __restrict__ y_inv, const int* __restrict__ flags, int IW,
+                                                 int IH, bf16* __restrict__ xo, float* __restrict__ yo) {
+    const int n = blockIdx.y;
+    const int cell = blockIdx.x * 256 + threadIdx.x;
+    const int f = flags[n];
+    if (cell < IW * IH) {
+        int cx = cell % IW, cy = cell / IW;
+        int sx = (f & 1) ? IW - 1 - cx : cx, sy = (f & 2) ? IH - 1 - cy : cy;
+        const bf16* src = x + ((size_t)n * IW * IH + (size_t)sy * IW + sx) * 4;
+        bf16* dst = xo + ((size_t)n * IW * IH + cell) * 4;
+        if (f & 4) { dst[0] = src[2]; dst[1] = src[3]; dst[2] = src[0]; dst[3] = src[1]; }
+        else { dst[0] = src[0]; dst[1] = src[1]; dst[2] = src[2]; dst[3] = src[3]; }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const float* b = (f & 4) ? y_inv + 6 * n : y + 6 * n;
+        float v[6] = {b[0], b[1], b[2], b[3], b[4], b[5]};
+        if (f & 1) { v[1] = -v[1]; v[2] = -v[2]; v[3] = -v[3]; }
+        if (f & 2) { v[0] = -v[0]; v[2] = -v[2]; v[4] = -v[4]; }
+        for (int k = 0; k < 6; k++) yo[6 * n + k] = v[k];
+    }
+}
+
+// Trel = [R(w) | t] from the scaled outputs (Rodrigues), FP64, stored FP32
+__global__ void k_outputs_to_trel(const float* __restrict__ out, int n, float rot_scale_rad, float trans_scale,
+                                  float* __restrict__ trel) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double w[3], t[3];
+    for (int k = 0; k < 3; k++) { w[k] = (double)out[6 * i + k] * rot_scale_rad; t[k] = (double)out[6 * i + 3 + k] * trans_scale; }
+    double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+    double A = th > 1e-12 ? sin(th) / th : 1.0;
+    double B = th > 1e-12 ? (1.0 - cos(th)) / (th * th) : 0.5;
+    double K[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+    double K2[9];
+    for (int r = 0; r < 3; r++)
+        for (int c = 0; c < 3; c++) {
+            double s = 0.0;
+            for (int q = 0; q < 3; q++) s += K[3 * r + q] * K[3 * q + c];
+            K2[3 * r + c] = s;
+        }
+    float* T = trel + 16 * (size_t)i;
+    for (int r = 0; r < 3; r++) {
+        for (int c = 0; c < 3; c++) T[4 * r + c] = (float)((r == c ? 1.0 : 0.0) + A * K[3 * r + c] + B * K2[3 * r + c]);
+        T[4 * r + 3] = (float)t[r];
+    }
+    T[12] = 0.0f; T[13] = 0.0f; T[14] = 0.0f; T[15] = 1.0f;
+}
+
+}  // namespace rt
+
+using namespace rt;
+
+// ---------------------------------------------------------------- host object
+struct rt_cnn {
+    int device = 0, max_batch = 0;
+    rt_cnn_config cfg;
+    hipStream_t stream = nullptr;
+    int H[7], W[7];
+    size_t w_off[6], b_off[6], fc_w, fc_b, nparams;
+    int K[6], Kp[6];
+    std::vector<void*> allocs;
+    float *params = nullptr, *grads = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+    bf16* Wp[6] = {};
+    bf16* WT[6] = {};
+    bf16* act[7] = {};          // act[0] is the caller's input during a call
+    bf16 *dz_a = nullptr, *dz_b = nullptr;
+    bf16* xin = nullptr;        // inputs built by rt_batch_propose
+    float *ws = nullptr, *ws_b = nullptr, *ws_fc = nullptr, *ws_loss = nullptr, *out = nullptr, *loss = nullptr;
+    size_t ws_floats = 0, ws_b_floats = 0;
+    FrameSrc* src_dev = nullptr;
+    int* flags_dev = nullptr;
+    float* trel_tmp = nullptr;
+    int step = 0;
+    bool packed = false;
+};
+
+static int ilog2(int v) {
+    int s = 0;
+    while ((1 << s) < v) s++;
+    return (1 << s) == v ? s : -1;
+}
+
+extern "C" void rt_cnn_default_config(rt_cnn_config* c) {
+    if (!c) return;
+    c->in_width = 80; c->in_height = 60;
+    const int ch[7] = {4, 32, 64, 128, 128, 256, 256};
+    for (int l = 0; l < 7; l++) c->channels[l] = ch[l];
+    c->rot_scale_deg = 10.0f;
+    c->trans_scale_mm = 150.0f;
+}
+
+static ConvGeom geom(const rt_cnn* net, int l, int B) {
+    ConvGeom g;
+    g.B = B; g.Hi = net->H[l]; g.Wi = net->W[l]; g.Ci = net->cfg.channels[l]; g.ci_shift = ilog2(g.Ci);
+    g.Ho = net->H[l + 1]; g.Wo = net->W[l + 1]; g.Co = net->cfg.channels[l + 1]; g.co_shift = ilog2(g.Co);
+    g.K = net->K[l]; g.Kp = net->Kp[l];
+    return g;
+}
+
+static size_t act_elems(const rt_cnn* net, int l, int B) {
+    return (size_t)B * net->H[l] * net->W[l] * net->cfg.channels[l];
+}
+
+// weight-gradient split: chunks of output pixels (multiple of 16) so that the
+// grid holds about 2048 waves
+static void wsplit(const rt_cnn* net, int l, int B, int* S, int* chunk) {
+    int M = B * net->H[l + 1] * net->W[l + 1];
+    int tiles = ((net->cfg.channels[l + 1] + 31) / 32) * (net->Kp[l] / 32);
+    int s = std::max(1, std::min(2048 / std::max(tiles, 1), (M + 15) / 16));
+    int c = ((M + s - 1) / s + 15) / 16 * 16;
+    *chunk = c;
+    *S = (M + c - 1) / c;
+}
+
+extern "C" int rt_cnn_create(int device, const rt_cnn_config* c, int max_batch, rt_cnn** out) {
+    if (!out || max_batch < 1) return set_error(RT_ERR_ARG, "rt_cnn_create: bad argument");
+    *out = nullptr;
+    rt_cnn_config cfg;
+    if (c) cfg = *c; else rt_cnn_default_config(&cfg);
+    if (cfg.in_width < 2 || cfg.in_height < 2) return set_error(RT_ERR_ARG, "rt_cnn_create: input too small");
+    if (cfg.channels[0] != 4) return set_error(RT_ERR_ARG, "rt_cnn_create: the input has 4 channels");
+    for (int l = 1; l < 7; l++)
+        if (ilog2(cfg.channels[l]) < 4 || cfg.channels[l] > 1024)
+            return set_error(RT_ERR_ARG, "rt_cnn_create: conv channels must be powers of two in 16..1024");
+    if (!(cfg.rot_scale_deg > 0) || !(cfg.trans_scale_mm > 0)) return set_error(RT_ERR_ARG, "rt_cnn_create: bad output scales");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return set_error(RT_ERR_ARG, "no such device");
+    if (hipSetDevice(device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    rt_cnn* net = new rt_cnn();
+    net->device = device; net->max_batch = max_batch; net->cfg = cfg;
+    net->H[0] = cfg.in_height; net->W[0] = cfg.in_width;
+    for (int l = 1; l < 7; l++) { net->H[l] = (net->H[l - 1] + 1) / 2; net->W[l] = (net->W[l - 1] + 1) / 2; }
+    size_t o = 0;
+    for (int l = 0; l < 6; l++) {
+        net->K[l] = 9 * cfg.channels[l];
+        net->Kp[l] = (net->K[l] + 31) / 32 * 32;
+        net->w_off[l] = o; o += (size_t)cfg.channels[l + 1] * net->K[l];
+        net->b_off[l] = o; o += (size_t)cfg.channels[l + 1];
+    }
+    net->fc_w = o; o += 6 * (size_t)cfg.channels[6];
+    net->fc_b = o; o += 6;
+    net->nparams = o;
+    // workspace sizes
+    size_t wsf = 0, wsb = 0, maxact = 0;
+    for (int l = 0; l < 6; l++) {
+        // wsplit never uses more than 2048 / tiles splits, whatever the batch
+        int tiles = ((cfg.channels[l + 1] + 31) / 32) * (net->Kp[l] / 32);
+        int S = std::max(1, 2048 / std::max(tiles, 1));
+        wsf = std::max(wsf, (size_t)S * cfg.channels[l + 1] * net->Kp[l]);
+        int M = max_batch * net->H[l + 1] * net->W[l + 1];
+        int Sb = std::min(1024, (M + 255) / 256);
+        wsb = std::max(wsb, (size_t)Sb * cfg.channels[l + 1]);
+    }
+    for (int l = 1; l < 7; l++) maxact = std::max(maxact, act_elems(net, l, max_batch));
+    net->ws_floats = wsf; net->ws_b_floats = wsb;
+    bool ok = hipStreamCreateWithFlags(&net->stream, hipStreamNonBlocking) == hipSuccess;
+    auto al = [&](void** p, size_t bytes) {
+        if (!ok) return;
+        if (hipMalloc(p, std::max(bytes, (size_t)256)) != hipSuccess) { ok = false; return; }
+        net->allocs.push_back(*p);
+        if (hipMemset(*p, 0, std::max(bytes, (size_t)256)) != hipSuccess) ok = false;
+    };
+    al((void**)&net->params, 4 * o); al((void**)&net->grads, 4 * o);
+    al((void**)&net->adam_m, 4 * o); al((void**)&net->adam_v, 4 * o);
+    for (int l = 0; l < 6; l++) {
+        al((void**)&net->Wp[l], 2 * (size_t)cfg.channels[l + 1] * net->Kp[l]);
+        if (l > 0) al((void**)&net->WT[l], 2 * (size_t)cfg.channels[l + 1] * net->K[l]);
+    }
+    for (int l = 1; l < 7; l++) al((void**)&net->act[l], 2 * act_elems(net, l, max_batch));
+    al((void**)&net->dz_a, 2 * maxact); al((void**)&net->dz_b, 2 * maxact);
+    al((void**)&net->xin, 2 * act_elems(net, 0, max_batch));
+    al((void**)&net->ws, 4 * wsf); al((void**)&net->ws_b, 4 * wsb);
+    al((void**)&net->ws_fc, 4 * (size_t)max_batch * (6 * cfg.channels[6] + 6));
+    al((void**)&net->ws_loss, 4 * (size_t)max_batch);
+    al((void**)&net->out, 4 * 6 * (size_t)max_batch);
+    al((void**)&net->loss, 256);
+    al((void**)&net->src_dev, sizeof(FrameSrc) * (size_t)max_batch);
+    al((void**)&net->flags_dev, 4 * (size_t)max_batch);
+    if (!ok) {
+        for (void* p : net->allocs) hipFree(p);
+        if (net->stream) hipStreamDestroy(net->stream);
+        delete net;
+        return set_error(RT_ERR_NOMEM, "rt_cnn_create: device allocation");
+    }
+    *out = net;
+    return RT_OK;
+}
+
+extern "C" void rt_cnn_destroy(rt_cnn* net) {
+    if (!net) return;
+    hipSetDevice(net->device);
+    hipStreamSynchronize(net->stream);
+    for (void* p : net->allocs) hipFree(p);
+    hipStreamDestroy(net->stream);
+    delete net;
+}
+
+extern "C" size_t rt_cnn_num_params(const rt_cnn* net) { return net ? net->nparams : 0; }
+extern "C" void* rt_cnn_stream(rt_cnn* net) { return net ? (void*)net->stream : nullptr; }
+
+static int pack(rt_cnn* net, hipStream_t s) {
+    for (int l = 0; l < 6; l++) {
+        int Co = net->cfg.channels[l + 1], Ci = net->cfg.channels[l];
+        int n = Co * net->Kp[l];
+        k_pack<<<(n + 255) / 256, 256, 0, s>>>(net->params + net->w_off[l], Co, Ci, net->Kp[l], net->Wp[l], net->WT[l]);
+    }
+    if (hipGetLastError() != hipSuccess) return set_error(RT_ERR_HIP, "pack launch");
+    net->packed = true;
+    return RT_OK;
+}
+
+extern "C" int rt_cnn_set_params(rt_cnn* net, const float* params) {
+    if (!net || !params) return set_error(RT_ERR_ARG, "rt_cnn_set_params: null argument");
+    if (hipSetDevice(net->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    size_t b = 4 * net->nparams;
+    if (hipMemcpyAsync(net->params, params, b, hipMemcpyHostToDevice, net->stream) != hipSuccess ||
+        hipMemsetAsync(net->adam_m, 0, b, net->stream) != hipSuccess || hipMemsetAsync(net->adam_v, 0, b, net->stream) != hipSuccess)
+        return set_error(RT_ERR_HIP, "rt_cnn_set_params: copy");
+    net->step = 0;
+    int rc = pack(net, net->stream);
+    if (rc) return rc;
+    if (hipStreamSynchronize(net->stream) != hipSuccess) return set_error(RT_ERR_HIP, "rt_cnn_set_params: sync");
+    return RT_OK;
+}
+
+extern "C" int rt_cnn_get_params(rt_cnn* net, float* params) {
+    if (!net || !params) return set_error(RT_ERR_ARG, "rt_cnn_get_params: null argument");
+    if (hipSetDevice(net->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    if (hipMemcpyAsync(params, net->params, 4 * net->nparams, hipMemcpyDeviceToHost, net->stream) != hipSuccess ||
+        hipStreamSynchronize(net->stream) != hipSuccess)
+        return set_error(RT_ERR_HIP, "rt_cnn_get_params: copy");
+    return RT_OK;
+}
+
+static int forward(rt_cnn* net, int n, const bf16* x, hipStream_t s) {
+    if (!net->packed) return set_error(RT_ERR_STATE, "rt_cnn: parameters not set");
+    net->act[0] = const_cast<bf16*>(x);
+    for (int l = 0; l < 6; l++) {
+        ConvGeom g = geom(net, l, n);
+        int M = n * g.Ho * g.Wo;
+        dim3 grid((M + 63) / 64, (g.Co + 63) / 64);
+        k_conv_fwd<<<grid, 256, 0, s>>>(g, net->act[l], net->Wp[l], net->params + net->b_off[l], net->act[l + 1]);
+    }
+    return hipGetLastError() == hipSuccess ? RT_OK : set_error(RT_ERR_HIP, "rt_cnn forward launch");
+}
+
+// forward + backward into net->grads; loss into net->loss[0] (device)
+static int loss_grad(rt_cnn* net, int n, const bf16* x, const float* y, const float* wn, hipStream_t s) {
+    int rc = forward(net, n, x, s);
+    if (rc) return rc;
+    k_weight_norm<<<1, 64, 0, s>>>(wn, n, net->loss + 1);
+    const int C6 = net->cfg.channels[6], P6 = net->H[6] * net->W[6];
+    bf16* dz = net->dz_a;
+    bf16* dz_other = net->dz_b;
+    k_head<<<n, 256, 2 * C6 * 4, s>>>(P6, C6, net->act[6], net->params + net->fc_w, net->params + net->fc_b, y, wn,
+                                      net->loss + 1, net->out, dz, net->ws_fc, net->ws_loss);
+    int width = 6 * C6 + 6;
+    k_reduce_rows<<<(width + 255) / 256, 256, 0, s>>>(net->ws_fc, n, width, net->grads + net->fc_w);
+    k_reduce_rows<<<1, 256, 0, s>>>(net->ws_loss, n, 1, net->loss);
+    for (int l = 5; l >= 0; l--) {
+        ConvGeom g = geom(net, l, n);
+        int M = n * g.Ho * g.Wo;
+        int S, chunk;
+        wsplit(net, l, n, &S, &chunk);
+        dim3 gw((g.Co + 31) / 32, g.Kp / 32, S);
+        k_conv_bwd_w<<<gw, 64, 0, s>>>(g, dz, net->act[l], net->ws, chunk);
+        int nw = g.Co * g.K;
+        k_reduce_w<<<(nw + 255) / 256, 256, 0, s>>>(net->ws, S, g.Co, g.Kp, g.K, net->grads + net->w_off[l]);
+        int Sb = std::min(1024, (M + 255) / 256);
+        int cb = (M + Sb - 1) / Sb;
+        Sb = (M + cb - 1) / cb;
+        k_colsum<<<dim3((g.Co + 255) / 256, Sb), 256, 0, s>>>(dz, M, g.Co, cb, net->ws_b);
+        k_reduce_rows<<<(g.Co + 255) / 256, 256, 0, s>>>(net->ws_b, Sb, g.Co, net->grads + net->b_off[l]);
+        if (l > 0) {
+            int Mi = n * g.Hi * g.Wi;
+            dim3 gd((Mi + 63) / 64, (g.Ci + 63) / 64);
+            k_conv_bwd_data<<<gd, 256, 0, s>>>(g, dz, net->WT[l], net->act[l], dz_other);
+            std::swap(dz, dz_other);
+        }
+    }
+    return hipGetLastError() == hipSuccess ? RT_OK : set_error(RT_ERR_HIP, "rt_cnn backward launch");
+}
+
+static int check_batch(rt_cnn* net, int n, const void* x) {
+    if (!net || !x || n < 1 || n > net->max_batch) return set_error(RT_ERR_ARG, "rt_cnn: bad batch");
+    if (hipSetDevice(net->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    return RT_OK;
+}
+
+extern "C" int rt_cnn_forward(rt_cnn* net, int n, const uint16_t* x_dev, float* out_dev) {
+    int rc = check_batch(net, n, x_dev);
+    if (rc) return rc;
+    if (!out_dev) return set_error(RT_ERR_ARG, "rt_cnn_forward: null output");
+    if ((rc = forward(net, n, x_dev, net->stream))) return rc;
+    const int C6 = net->cfg.channels[6], P6 = net->H[6] * net->W[6];
+    k_head<<<n, 256, 2 * C6 * 4, net->stream>>>(P6, C6, net->act[6], net->params + net->fc_w, net->params + net->fc_b,
+                                                nullptr, nullptr, nullptr, out_dev, nullptr, nullptr, nullptr);
+    if (hipGetLastError() != hipSuccess) return set_error(RT_ERR_HIP, "rt_cnn head launch");
+    return RT_OK;
+}
+
+extern "C" int rt_cnn_loss_grad(rt_cnn* net, int n, const uint16_t* x_dev, const float* y_dev, const float* w_dev,
+                                float* loss, float* out_host, float* grad_host) {
+    int rc = check_batch(net, n, x_dev);
+    if (rc) return rc;
+    if (!y_dev || !w_dev) return set_error(RT_ERR_ARG, "rt_cnn_loss_grad: null targets");
+    if ((rc = loss_grad(net, n, x_dev, y_dev, w_dev, net->stream))) return rc;
+    bool ok = true;
+    if (loss) ok = ok && hipMemcpyAsync(loss, net->loss, 4, hipMemcpyDeviceToHost, net->stream) == hipSuccess;
+    if (out_host) ok = ok && hipMemcpyAsync(out_host, net->out, 24 * (size_t)n, hipMemcpyDeviceToHost, net->stream) == hipSuccess;
+    if (grad_host) ok = ok && hipMemcpyAsync(grad_host, net->grads, 4 * net->nparams, hipMemcpyDeviceToHost, net->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(net->stream) == hipSuccess;
+    return ok ? RT_OK : set_error(RT_ERR_HIP, "rt_cnn_loss_grad: copy");
+}
+
+extern "C" int rt_cnn_train_step(rt_cnn* net, int n, const uint16_t* x_dev, const float* y_dev, const float* w_dev,
+                                 float lr, float* loss) {
+    int rc = check_batch(net, n, x_dev);
+    if (rc) return rc;
+    if (!y_dev || !w_dev || !(lr >= 0)) return set_error(RT_ERR_ARG, "rt_cnn_train_step: bad argument");
+    if ((rc = loss_grad(net, n, x_dev, y_dev, w_dev, net->stream))) return rc;
+    net->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    float c1 = (float)(1.0 - pow(0.9, net->step)), c2 = (float)(1.0 - pow(0.999, net->step));
+    size_t P = net->nparams;
+    k_adam<<<(unsigned)((P + 255) / 256), 256, 0, net->stream>>>(net->params, net->grads, net->adam_m, net->adam_v, P, lr, b1,
+                                                                  b2, c1, c2, 1e-8f);
+    if (hipGetLastError() != hipSuccess) return set_error(RT_ERR_HIP, "rt_cnn adam launch");
+    if ((rc = pack(net, net->stream))) return rc;
+    if (loss) {
+        if (hipMemcpyAsync(loss, net->loss, 4, hipMemcpyDeviceToHost, net->stream) != hipSuccess ||
+            hipStreamSynchronize(net->stream) != hipSuccess)
+            return set_error(RT_ERR_HIP, "rt_cnn_train_step: loss copy");
+    }
+    return RT_OK;
+}
+
+extern "C" int rt_cnn_augment(rt_cnn* net, int n, const uint16_t* x_dev, const float* y_dev, const float* y_inv_dev,
+                              const int* flags, uint16_t* x_out, float* y_out) {
+    int rc = check_batch(net, n, x_dev);
+    if (rc) return rc;
+    if (!y_dev || !y_inv_dev || !flags || !x_out || !y_out) return set_error(RT_ERR_ARG, "rt_cnn_augment: null argument");
+    if (net->cfg.channels[0] != 4) return set_error(RT_ERR_UNSUPPORTED, "rt_cnn_augment: 4-channel inputs only");
+    if (hipMemcpyAsync(net->flags_dev, flags, 4 * (size_t)n, hipMemcpyHostToDevice, net->stream) != hipSuccess)
+        return set_error(RT_ERR_HIP, "rt_cnn_augment: flags copy");
+    int cells = net->cfg.in_width * net->cfg.in_height;
+    k_augment<<<dim3((cells + 255) / 256, n), 256, 0, net->stream>>>(x_dev, y_dev, y_inv_dev, net->flags_dev, net->cfg.in_width,
+                                                                     net->cfg.in_height, x_out, y_out);
+    return hipGetLastError() == hipSuccess ? RT_OK : set_error(RT_ERR_HIP, "rt_cnn_augment launch");
+}
+
+static int make_inputs(rt_cnn* net, int n, const FrameSrc* src, bf16* x, hipStream_t s) {
+    for (int i = 0; i < n; i++) {
+        const FrameSrc& f = src[i];
+        if (!f.g8[0] || !f.g8[1] || !f.d16[0] || !f.d16[1]) return set_error(RT_ERR_ARG, "rt_cnn: null frame");
+        if (f.W < net->cfg.in_width || f.H < net->cfg.in_height || f.W % net->cfg.in_width || f.H % net->cfg.in_height)
+            return set_error(RT_ERR_ARG, "rt_cnn: frame size must be a multiple of the network input size");
+    }
+    if (hipMemcpyAsync(net->src_dev, src, sizeof(FrameSrc) * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess)
+        return set_error(RT_ERR_HIP, "rt_cnn: frame table copy");
+    int cells = net->cfg.in_width * net->cfg.in_height;
+    k_inputs<<<dim3((cells + 255) / 256, n), 256, 0, s>>>(net->src_dev, net->cfg.in_width, net->cfg.in_height, x);
+    return hipGetLastError() == hipSuccess ? RT_OK : set_error(RT_ERR_HIP, "rt_cnn inputs launch");
+}
+
+extern "C" int rt_cnn_make_inputs(rt_cnn* net, int n, const rt_pair* pairs, uint16_t* x_dev) {
+    int rc = check_batch(net, n, x_dev);
+    if (rc) return rc;
+    if (!pairs) return set_error(RT_ERR_ARG, "rt_cnn_make_inputs: null pairs");
+    std::vector<FrameSrc> src(n);
+    for (int i = 0; i < n; i++) {
+        src[i].W = pairs[i].width; src[i].H = pairs[i].height;
+        src[i].g8[0] = pairs[i].ref_gray; src[i].g8[1] = pairs[i].cur_gray;
+        src[i].d16[0] = pairs[i].ref_depth; src[i].d16[1] = pairs[i].cur_depth;
+    }
+    if ((rc = make_inputs(net, n, src.data(), x_dev, net->stream))) return rc;
+    return hipStreamSynchronize(net->stream) == hipSuccess ? RT_OK : set_error(RT_ERR_HIP, "rt_cnn_make_inputs: sync");
+}
+
+namespace rt {
+int cnn_propose(rt_cnn* net, int n, const FrameSrc* src, float* trel_dev, hipStream_t s) {
+    if (!net || n < 1 || n > net->max_batch) return set_error(RT_ERR_ARG, "rt_batch_propose: batch larger than the network's max_batch");
+    if (!net->packed) return set_error(RT_ERR_STATE, "rt_batch_propose: network parameters not set");
+    if (hipSetDevice(net->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    bf16* x = net->xin;
+    int rc = make_inputs(net, n, src, x, s);
+    if (rc) return rc;
+    if ((rc = forward(net, n, x, s))) return rc;
+    const int C6 = net->cfg.channels[6], P6 = net->H[6] * net->W[6];
+    k_head<<<n, 256, 2 * C6 * 4, s>>>(P6, C6, net->act[6], net->params + net->fc_w, net->params + net->fc_b, nullptr,
+                                      nullptr, nullptr, net->out, nullptr, nullptr, nullptr);
+    k_outputs_to_trel<<<(n + 255) / 256, 256, 0, s>>>(net->out, n, net->cfg.rot_scale_deg * (float)(M_PI / 180.0),
+                                                      net->cfg.trans_scale_mm, trel_dev);
+    return hipGetLastError() == hipSuccess ? RT_OK : set_error(RT_ERR_HIP, "rt_batch_propose launch");
+}
+}  // namespace rt
+
+// ---------------------------------------------------------------- host pose conversions
+static void log_rotation(const double* T, double w[3]) {
+    // rotation vector of the 3x3 block of a row-major 4x4
+    double R[9] = {T[0], T[1], T[2], T[4], T[5], T[6], T[8], T[9], T[10]};
+    double sx = R[7] - R[5], sy = R[2] - R[6], sz = R[3] - R[1];
+    double s = 0.5 * sqrt(sx * sx + sy * sy + sz * sz);
+    double c = 0.5 * (R[0] + R[4] + R[8] - 1.0);
+    double th = atan2(s, c);
+    if (th < 1e-9) { w[0] = 0.5 * sx; w[1] = 0.5 * sy; w[2] = 0.5 * sz; return; }
+    if (M_PI - th < 1e-6) {
+        // near pi: axis from the diagonal
+        double ax[3] = {sqrt(std::max(0.0, (R[0] + 1) / 2)), sqrt(std::max(0.0, (R[4] + 1) / 2)), sqrt(std::max(0.0, (R[8] + 1) / 2))};
+        if (R[1] < 0) ax[1] = -ax[1];
+        if (R[2] < 0) ax[2] = -ax[2];
+        for (int k = 0; k < 3; k++) w[k] = th * ax[k];
+        return;
+    }
+    double f = th / (2.0 * s);
+    w[0] = f * sx; w[1] = f * sy; w[2] = f * sz;
+}
+
+extern "C" int rt_cnn_pose_to_target(const rt_cnn_config* c, int n, const double* Trel, float* y) {
+    if (!c || !Trel || !y || n < 0) return set_error(RT_ERR_ARG, "rt_cnn_pose_to_target: bad argument");
+    double rs = c->rot_scale_deg * M_PI / 180.0;
+    for (int i = 0; i < n; i++) {
+        const double* T = Trel + 16 * (size_t)i;
+        double w[3];
+        log_rotation(T, w);
+        for (int k = 0; k < 3; k++) {
+            y[6 * i + k] = (float)(w[k] / rs);
+            y[6 * i + 3 + k] = (float)(T[4 * k + 3] / c->trans_scale_mm);
+        }
+    }
+    return RT_OK;
+}
+
+extern "C" int rt_cnn_target_to_pose(const rt_cnn_config* c, int n, const float* y, double* Trel) {
+    if (!c || !Trel || !y || n < 0) return set_error(RT_ERR_ARG, "rt_cnn_target_to_pose: bad argument");
+    double rs = c->rot_scale_deg * M_PI / 180.0;
+    for (int i = 0; i < n; i++) {
+        double w[3], t[3];
+        for (int k = 0; k < 3; k++) { w[k] = y[6 * i + k] * rs; t[k] = (double)y[6 * i + 3 + k] * c->trans_scale_mm; }
+        double th = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        double A = th > 1e-12 ? sin(th) / th : 1.0, B = th > 1e-12 ? (1.0 - cos(th)) / (th * th) : 0.5;
+        double K[9] = {0, -w[2], w[1], w[2], 0, -w[0], -w[1], w[0], 0};
+        double* T = Trel + 16 * (size_t)i;
+        for (int r = 0; r < 3; r++) {
+            for (int q = 0; q < 3; q++) {
+                double k2 = 0.0;
+                for (int m = 0; m < 3; m++) k2 += K[3 * r + m] * K[3 * m + q];
+                T[4 * r + q] = (r == q ? 1.0 : 0.0) + A * K[3 * r + q] + B * k2;
+            }
+            T[4 * r + 3] = t[r];
+        }
+        T[12] = 0; T[13] = 0; T[14] = 0; T[15] = 1;
+    }
+    return RT_OK;
+}

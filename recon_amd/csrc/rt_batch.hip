@@ -22,6 +22,7 @@
 #include <string>
 #include <vector>
 #include "../../include/rt_tracker.h"
+#include "cnn.hpp"
 #include "rt_kernels.hpp"
 #include "rt_prep.hpp"
 #include "rt_host.hpp"
@@ -539,8 +540,8 @@ static int run_track(rt_batch* b, rt_pair_result* out) {
     return out ? fetch_results(b, out) : RT_OK;
 }
 
-extern "C" int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_result* out) {
-    if (!b || !pairs || n < 1 || n > b->max_pairs) return set_error(RT_ERR_ARG, "rt_batch_track: bad argument");
+extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
+    if (!b || !pairs || n < 1 || n > b->max_pairs) return set_error(RT_ERR_ARG, "rt_batch_prepare: bad argument");
     if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
     std::vector<PairDesc> desc(n);
     std::vector<float> calib(96 * (size_t)n);
@@ -606,7 +607,25 @@ extern "C" int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_
         if (rc) return rc;
     }
     if (hipGraphLaunch(b->graph_prep, b->stream) != hipSuccess) return set_error(RT_ERR_HIP, "graph launch");
-    return run_track(b, out);
+    return RT_OK;
+}
+
+extern "C" int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_result* out) {
+    int rc = rt_batch_prepare(b, n, pairs);
+    return rc ? rc : run_track(b, out);
+}
+
+extern "C" int rt_batch_propose(rt_batch* b, rt_cnn* net) {
+    if (!b || !net || b->n < 1) return set_error(RT_ERR_STATE, "rt_batch_propose: no prepared batch or no network");
+    if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    std::vector<FrameSrc> src(b->n);
+    for (int i = 0; i < b->n; i++) {
+        const PairDesc& d = b->desc[i];
+        src[i].W = d.W; src[i].H = d.H;
+        src[i].g8[0] = d.g8[0]; src[i].g8[1] = d.g8[1];
+        src[i].d16[0] = d.d16[0]; src[i].d16[1] = d.d16[1];
+    }
+    return cnn_propose(net, b->n, src.data(), b->trel_in, b->stream);
 }
 
 // out == NULL: enqueue the launch and return without waiting (timing).

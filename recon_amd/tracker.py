@@ -235,6 +235,10 @@ class BatchTracker:
         """pairs: list of dicts with device pointers ref_gray, ref_depth, cur_gray,
         cur_depth (ints), width, height, calib (np.float32[96]); or the result
         of prepare()."""
+        if isinstance(pairs, tuple) and init is not None:
+            # a prepared array carries its own initial poses: prepare, then track from init
+            self.prepare_only(pairs)
+            return self.retrack(np.stack([np.asarray(t, np.float64).reshape(4, 4) for t in init]), pairs[1])
         prep = pairs if isinstance(pairs, tuple) else self.prepare(pairs, init)
         arr, n, out, _keep = prep
         check(self.L.rt_batch_track(self.h, n, arr, out), "rt_batch_track")
@@ -258,6 +262,15 @@ class BatchTracker:
                 arr[i].init_T = t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
         out = (RtPairResult * n)()
         return (arr, n, out, keep)
+
+    def prepare_only(self, prepared):
+        """Keyframe preparation of the pairs of prepare() without tracking."""
+        arr, n, _out, _keep = prepared
+        check(self.L.rt_batch_prepare(self.h, n, arr), "rt_batch_prepare")
+
+    def propose(self, net):
+        """Initial poses of the prepared pairs from the pose network (PoseNet)."""
+        check(self.L.rt_batch_propose(self.h, net.h), "rt_batch_propose")
 
     def retrack(self, init=None, n=None):
         out = (RtPairResult * n)()
