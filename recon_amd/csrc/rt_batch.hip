@@ -174,7 +174,13 @@ struct TrackCfg {
     float scale_out;
 };
 
-template <int BS, int PPT>
+// wave-uniform copy of a value (all lanes hold the same): lives in a scalar register
+__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+
+// RZ: residual and zWeight of each point kept in registers (0) or in LDS (1,
+// at most RZ_MAX points per pair), trading LDS for occupancy.
+constexpr int RZ_MAX = 8192;
+template <int BS, int PPT, int RZ>
 __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, const int* __restrict__ order,
                                                int npairs, int* queue, TrackCfg cfg) {
     constexpr int NW = BS / 64;
@@ -182,9 +188,14 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
     __shared__ float T[16];
     __shared__ unsigned hist[256];
     __shared__ double wsum[NW][29];
+    __shared__ double sys[29];
+    __shared__ double ws[SOLVE_WS];
     __shared__ int s_pair, s_med;
     __shared__ int s_ok;
     __shared__ float lut[256];
+    // RZ: dynamic LDS after every static array, so that the static arrays keep
+    // small addresses (immediate offsets from one base register)
+    extern __shared__ float2 rz_lds[];
     const int t = threadIdx.x, wave = t >> 6, lane = lane_id();
     load_lut(lut);
     for (;;) {
@@ -194,7 +205,7 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
             s_pair = q < npairs ? order[q] : -1;
         }
         __syncthreads();
-        int p = s_pair;
+        const int p = __builtin_amdgcn_readfirstlane(s_pair);
         if (p < 0) break;
         const PairDesc& d = P[p];
         const int W = d.W, H = d.H, L = d.L, N = d.N;
@@ -202,6 +213,10 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
         if (t < 16) T[t] = d.trel_in[t];
         if (t == 0) s_ok = 1;
         __syncthreads();
+        // calibration and pose are the same for every lane: scalar registers
+        float cl[96];
+#pragma unroll
+        for (int k = 0; k < 96; k++) cl[k] = uni(cal[k]);
         float rms_last = 0.0f;
         for (int layer = L - 1; layer >= 0; layer--) {
             const int nit = d.iters[L - 1 - layer];
@@ -213,23 +228,33 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
                 for (int k = t; k < 256; k += BS) hist[k] = 0u;
                 float Tl[16];
 #pragma unroll
-                for (int k = 0; k < 16; k++) Tl[k] = T[k];
+                for (int k = 0; k < 16; k++) Tl[k] = uni(T[k]);
                 __syncthreads();
-                // residual pass: r and zWeight stay in registers
-                float r[PPT], zw[PPT];
+                // residual pass: r and zWeight stay in registers (RZ 0) or go to LDS (RZ 1)
+                float r[RZ ? 1 : PPT], zw[RZ ? 1 : PPT];
+                auto residual = [&](int i, float& rr, float& zz) {
+                    if (layer == 0)
+                        photo_point(ldg(d.pts, i), ldg(jv.iref, i), Tl, a, b, U8Src{d.g8[1], lut}, wl, hl,
+                                    U16DepthSrc{d.d16[1]}, W, H, cl, rr, zz);
+                    else
+                        photo_point(ldg(d.pts, i), ldg(jv.iref, i), Tl, a, b, F32Src{img}, wl, hl,
+                                    U16DepthSrc{d.d16[1]}, W, H, cl, rr, zz);
+                    int bn = (int)((float)(cfg.mad_bins - 1) * fabsf(rr));
+                    if (bn >= 0 && bn < cfg.mad_bins) atomicAdd(&hist[bn], 1u);
+                };
+                if constexpr (RZ) {
+#pragma unroll PPT
+                    for (int i = t; i < N; i += BS) {
+                        float rr, zz;
+                        residual(i, rr, zz);
+                        rz_lds[i] = make_float2(rr, zz);
+                    }
+                } else {
 #pragma unroll
-                for (int k = 0; k < PPT; k++) {
-                    int i = t + k * BS;
-                    r[k] = 0.0f; zw[k] = 0.0f;
-                    if (i < N) {
-                        if (layer == 0)
-                            photo_point(d.pts[i], jv.iref[i], Tl, a, b, U8Src{d.g8[1], lut}, wl, hl,
-                                        U16DepthSrc{d.d16[1]}, W, H, cal, r[k], zw[k]);
-                        else
-                            photo_point(d.pts[i], jv.iref[i], Tl, a, b, F32Src{img}, wl, hl,
-                                        U16DepthSrc{d.d16[1]}, W, H, cal, r[k], zw[k]);
-                        int bn = (int)((float)(cfg.mad_bins - 1) * fabsf(r[k]));
-                        if (bn >= 0 && bn < cfg.mad_bins) atomicAdd(&hist[bn], 1u);
+                    for (int k = 0; k < PPT; k++) {
+                        int i = t + k * BS;
+                        r[k] = 0.0f; zw[k] = 0.0f;
+                        if (i < N) residual(i, r[k], zw[k]);
                     }
                 }
                 __syncthreads();
@@ -262,24 +287,33 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
                 float acc[29];
 #pragma unroll
                 for (int k = 0; k < 29; k++) acc[k] = 0.0f;
+                auto accumulate = [&](int i, float rk, float zk) {
+                    float w = robust_weight(rk, zk, s, cfg.robust);
+                    float wr = rk * w;
+                    float4 ja = ldg(jv.a, i);
+                    float2 jb = ldg(jv.b, i);
+                    float jw[6] = {ja.x * w, ja.y * w, ja.z * w, ja.w * w, jb.x * w, jb.y * w};
+                    int q = 0;
 #pragma unroll
-                for (int k = 0; k < PPT; k++) {
-                    int i = t + k * BS;
-                    if (i < N) {
-                        float w = robust_weight(r[k], zw[k], s, cfg.robust);
-                        float wr = r[k] * w;
-                        float4 ja = jv.a[i];
-                        float2 jb = jv.b[i];
-                        float jw[6] = {ja.x * w, ja.y * w, ja.z * w, ja.w * w, jb.x * w, jb.y * w};
-                        int q = 0;
+                    for (int r0 = 0; r0 < 6; r0++)
 #pragma unroll
-                        for (int r0 = 0; r0 < 6; r0++)
+                        for (int c0 = r0; c0 < 6; c0++) acc[q++] += jw[r0] * jw[c0];
 #pragma unroll
-                            for (int c0 = r0; c0 < 6; c0++) acc[q++] += jw[r0] * jw[c0];
+                    for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += jw[r0] * wr;
+                    acc[27] += wr * wr;
+                    acc[28] += w * w;
+                };
+                if constexpr (RZ) {
+#pragma unroll PPT
+                    for (int i = t; i < N; i += BS) {
+                        float2 v = rz_lds[i];
+                        accumulate(i, v.x, v.y);
+                    }
+                } else {
 #pragma unroll
-                        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += jw[r0] * wr;
-                        acc[27] += wr * wr;
-                        acc[28] += w * w;
+                    for (int k = 0; k < PPT; k++) {
+                        int i = t + k * BS;
+                        if (i < N) accumulate(i, r[k], zw[k]);
                     }
                 }
 #pragma unroll
@@ -288,23 +322,18 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
                     if (lane == 0) wsum[wave][k] = (double)v;
                 }
                 __syncthreads();
+                // wave sums combined in FP64 in wave order, one term per lane
+                if (t < 29) {
+                    double v = 0.0;
+                    for (int w2 = 0; w2 < NW; w2++) v += wsum[w2][t];
+                    sys[t] = v;
+                }
+                __syncthreads();
                 if (t == 0) {
-                    double sys[29];
-                    for (int k = 0; k < 29; k++) {
-                        double v = 0.0;
-                        for (int w2 = 0; w2 < NW; w2++) v += wsum[w2][k];
-                        sys[k] = v;
-                    }
-                    double A[36], bb[6], x[6];
-                    unpack27(sys, A, bb);
-                    bool ok = true;
-                    if (cfg.solver == 1) solve_cg(A, bb, x);
-                    else ok = solve_cholesky(A, bb, x);
-                    if (!ok) s_ok = 0;
-                    float TT[16];
-                    for (int k = 0; k < 16; k++) TT[k] = T[k];
-                    expm_update(x, TT, (double)cfg.scale_out);
-                    for (int k = 0; k < 16; k++) T[k] = TT[k];
+                    SolveCfg sc;
+                    sc.solver = cfg.solver; sc.mode = 1; sc.lambda = 0.0f; sc.scale_out = cfg.scale_out;
+                    sc.robust = cfg.robust; sc.mad_bins = cfg.mad_bins; sc.scale_in = 1e-3f;
+                    if (!solve_update(sys, T, sc, N, 1, ws)) s_ok = 0;
                     rms_last = sys[28] > 0.0 ? (float)sqrt(sys[27] / sys[28]) : 0.0f;
                 }
                 __syncthreads();
@@ -316,12 +345,15 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
 }
 
 typedef void (*TrackFn)(const PairDesc*, const int*, int, int*, TrackCfg);
-struct Variant { int bs, ppt; TrackFn fn; };
+struct Variant { int bs, ppt, rz; TrackFn fn; };
 static const Variant kVariants[] = {
-    {256, 8, kb_track<256, 8>},   {256, 16, kb_track<256, 16>}, {256, 32, kb_track<256, 32>},
-    {512, 4, kb_track<512, 4>},   {512, 8, kb_track<512, 8>},   {512, 16, kb_track<512, 16>},
-    {512, 32, kb_track<512, 32>}, {1024, 4, kb_track<1024, 4>}, {1024, 8, kb_track<1024, 8>},
-    {1024, 16, kb_track<1024, 16>}, {1024, 32, kb_track<1024, 32>},
+    {256, 8, 0, kb_track<256, 8, 0>},     {256, 16, 0, kb_track<256, 16, 0>},   {256, 32, 0, kb_track<256, 32, 0>},
+    {512, 4, 0, kb_track<512, 4, 0>},     {512, 8, 0, kb_track<512, 8, 0>},     {512, 16, 0, kb_track<512, 16, 0>},
+    {512, 32, 0, kb_track<512, 32, 0>},   {1024, 4, 0, kb_track<1024, 4, 0>},   {1024, 8, 0, kb_track<1024, 8, 0>},
+    {1024, 16, 0, kb_track<1024, 16, 0>}, {1024, 32, 0, kb_track<1024, 32, 0>},
+    // RZ variants: ppt is the unroll depth of a loop over the pair's points
+    {256, 2, 1, kb_track<256, 2, 1>},     {256, 4, 1, kb_track<256, 4, 1>},     {512, 2, 1, kb_track<512, 2, 1>},
+    {512, 4, 1, kb_track<512, 4, 1>},     {1024, 2, 1, kb_track<1024, 2, 1>},
 };
 
 }  // namespace rt
@@ -441,10 +473,25 @@ static TrackCfg track_cfg(const rt_batch* b) {
     return c;
 }
 
+static size_t track_dyn_lds(const Variant& v, int maxN) { return v.rz ? sizeof(float2) * (size_t)maxN : 0; }
+
+// RZ variants need more dynamic LDS than the default limit allows
+static bool rz_variant_ready(const Variant& v, int maxN) {
+    static bool tried[64] = {}, ok[64] = {};
+    int idx = (int)(&v - kVariants);
+    if (!tried[idx]) {
+        tried[idx] = true;
+        ok[idx] = hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      (int)(sizeof(float2) * RZ_MAX)) == hipSuccess;
+        (void)hipGetLastError();
+    }
+    return ok[idx] && maxN <= RZ_MAX;
+}
+
 static void launch_track(rt_batch* b, int variant, int grid, hipStream_t s) {
     kb_reset_queue<<<1, 64, 0, s>>>(b->queue);
     const Variant& v = kVariants[variant];
-    v.fn<<<grid, v.bs, 0, s>>>(b->desc_dev, b->order_dev, b->n, b->queue, track_cfg(b));
+    v.fn<<<grid, v.bs, track_dyn_lds(v, b->maxN), s>>>(b->desc_dev, b->order_dev, b->n, b->queue, track_cfg(b));
 }
 
 static std::string tune_key(const rt_batch* b) {
@@ -470,8 +517,12 @@ static int autotune(rt_batch* b) {
     int bv = -1, bg = 0;
     for (int v = 0; v < (int)(sizeof(kVariants) / sizeof(kVariants[0])); v++) {
         const Variant& var = kVariants[v];
-        if (var.bs * var.ppt < b->maxN) continue;
-        if (var.bs * var.ppt >= 4 * b->maxN && var.ppt > 4) continue;  // mostly idle lanes
+        if (var.rz) {
+            if (!rz_variant_ready(var, b->maxN)) continue;
+        } else {
+            if (var.bs * var.ppt < b->maxN) continue;
+            if (var.bs * var.ppt >= 4 * b->maxN && var.ppt > 4) continue;  // mostly idle lanes
+        }
         if (force && atoi(force) != v) continue;
         int per_cu_max = 2048 / var.bs;  // waves per CU stay within the hardware limit
         for (int k = 1; k <= 4 && k <= per_cu_max; k *= 2) {
@@ -491,7 +542,9 @@ static int autotune(rt_batch* b) {
     if (bv < 0) return set_error(RT_ERR_UNSUPPORTED, "no batch kernel variant covers the point count");
     b->tuned[key] = {bv, bg};
     b->variant = bv; b->grid = bg;
-    if (getenv("RT_VERBOSE")) fprintf(stderr, "rt_batch autotune %s -> bs %d ppt %d grid %d (%.3f ms)\n", key.c_str(), kVariants[bv].bs, kVariants[bv].ppt, bg, best);
+    if (getenv("RT_VERBOSE"))
+        fprintf(stderr, "rt_batch autotune %s -> bs %d ppt %d rz %d grid %d (%.3f ms)\n", key.c_str(), kVariants[bv].bs,
+                kVariants[bv].ppt, kVariants[bv].rz, bg, best);
     return RT_OK;
 }
 

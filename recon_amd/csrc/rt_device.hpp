@@ -57,20 +57,46 @@ __device__ __forceinline__ f2 distort(f2 pu, const float* kc, const float* K) {
 __device__ __forceinline__ bool in_bounds(f2 p, int w, int h) {
     return p.x >= 10.0f && p.x < (float)(w - 10) && p.y >= 10.0f && p.y < (float)(h - 10);
 }
+// Pointers into device memory (global address space): loads through them are
+// global_load with a scalar base, not flat loads, which would also wait on the
+// LDS counter and need 64-bit addresses per lane.
+#define RT_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ const RT_GLOBAL T* gmem(const T* p) { return (const RT_GLOBAL T*)p; }
+template <class T>
+__device__ __forceinline__ RT_GLOBAL T* gmem(T* p) { return (RT_GLOBAL T*)p; }
+
+typedef float nfloat4 __attribute__((ext_vector_type(4)));
+typedef float nfloat2 __attribute__((ext_vector_type(2)));
+// element i of a device array, loaded through the global address space
+template <class T>
+__device__ __forceinline__ T ldg(const T* p, size_t i) { return gmem(p)[i]; }
+__device__ __forceinline__ float4 ldg(const float4* p, size_t i) {
+    nfloat4 v = ((const RT_GLOBAL nfloat4*)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float2 ldg(const float2* p, size_t i) {
+    nfloat2 v = ((const RT_GLOBAL nfloat2*)p)[i];
+    return make_float2(v.x, v.y);
+}
+
 // Image sources: FP32 planes, or the raw inputs read through their exact
 // conversions: 8-bit gray as (float)g/255 from a 256-entry table, 16-bit depth
 // in mm as -(float)d (the values the conversion kernel would have stored).
 struct F32Src {
-    const float* __restrict__ p;
+    const RT_GLOBAL float* __restrict__ p;
+    __device__ F32Src(const float* q) : p(gmem(q)) {}
     __device__ __forceinline__ float operator[](size_t i) const { return p[i]; }
 };
 struct U8Src {
-    const uint8_t* __restrict__ p;
+    const RT_GLOBAL uint8_t* __restrict__ p;
     const float* lut;
+    __device__ U8Src(const uint8_t* q, const float* l) : p(gmem(q)), lut(l) {}
     __device__ __forceinline__ float operator[](size_t i) const { return lut[p[i]]; }
 };
 struct U16DepthSrc {
-    const uint16_t* __restrict__ p;
+    const RT_GLOBAL uint16_t* __restrict__ p;
+    __device__ U16DepthSrc(const uint16_t* q) : p(gmem(q)) {}
     __device__ __forceinline__ float operator[](size_t i) const { return -(float)p[i]; }
 };
 

@@ -246,7 +246,8 @@ __global__ __launch_bounds__(256) void k_photo_sys(const float2* __restrict__ re
     if (last_group(ticket)) {
         combine27(part, gridDim.x, sys);
         __syncthreads();
-        if (t == 0 && solve_here) solve_update(sys, trel, cfg, N, 1);
+        __shared__ double ws[SOLVE_WS];
+        if (t == 0 && solve_here) solve_update(sys, trel, cfg, N, 1, ws);
         // keep the histogram of this iteration (inspection) and clear it for the next
         madhist[256 + t] = madhist[t];
         madhist[t] = 0u;
@@ -581,7 +582,8 @@ __global__ __launch_bounds__(256) void k_icp_sys(const float4* __restrict__ rp, 
     if (last_group(ticket)) {
         combine27(part, gridDim.x, sys + 27);
         __syncthreads();
-        if (threadIdx.x == 0 && solve_here && n > 0) solve_update(sys, trel, cfg, N1, n);
+        __shared__ double ws[SOLVE_WS];
+        if (threadIdx.x == 0 && solve_here && n > 0) solve_update(sys, trel, cfg, N1, n, ws);
         if (threadIdx.x == 0) reset_ticket(ticket);
     }
 }

@@ -299,13 +299,25 @@ __device__ __forceinline__ void ref_point(int i, const ZS& z, const GS& g0,
 
 // ------------------------------------------------------------------ solve
 // conjugateGradientKernel (warp.cu:926-975) and a textbook FP64 Cholesky.
-__device__ void solve_cg(const double* A, const double* bb, double* x) {
-    double r[6], dir[6];
+// Every array the solve works on lives in a caller-provided workspace w (LDS
+// in the kernels), so that one lane's serial FP64 solve does not hold ~200
+// doubles in registers that the whole workgroup would then have to reserve.
+// Workspace layout (doubles): see SOLVE_WS.
+enum {
+    WS_A = 0, WS_B = 36, WS_X = 42, WS_AG = 48, WS_BG = 84, WS_SOLVE = 90, WS_EXPM = 132, SOLVE_WS = 216
+};
+__device__ void solve_cg(const double* A, const double* bb, double* x, double* w) {
+    double* r = w;
+    double* dir = w + 6;
+    double* Ad = w + 12;
     const double tol = 1e-8;
     for (int i = 0; i < 6; i++) { x[i] = 0.0; r[i] = bb[i]; dir[i] = bb[i]; }
     for (int step = 0; step < 6; step++) {
-        double Ad[6];
-        for (int j = 0; j < 6; j++) { Ad[j] = 0.0; for (int k = 0; k < 6; k++) Ad[j] += A[j * 6 + k] * dir[k]; }
+        for (int j = 0; j < 6; j++) {
+            double s = 0.0;
+            for (int k = 0; k < 6; k++) s += A[j * 6 + k] * dir[k];
+            Ad[j] = s;
+        }
         double rr = 0.0, Add = 0.0;
         for (int i = 0; i < 6; i++) rr += r[i] * r[i];
         for (int i = 0; i < 6; i++) Add += Ad[i] * dir[i];
@@ -321,8 +333,9 @@ __device__ void solve_cg(const double* A, const double* bb, double* x) {
         for (int i = 0; i < 6; i++) dir[i] = r[i] + beta * dir[i];
     }
 }
-__device__ bool solve_cholesky(const double* A, const double* b, double* x) {
-    double Lm[36];
+__device__ bool solve_cholesky(const double* A, const double* b, double* x, double* w) {
+    double* Lm = w;
+    double* y = w + 36;
     for (int i = 0; i < 36; i++) Lm[i] = 0.0;
     for (int j = 0; j < 6; j++) {
         double s = A[j * 6 + j];
@@ -336,7 +349,6 @@ __device__ bool solve_cholesky(const double* A, const double* b, double* x) {
             Lm[i * 6 + j] = t / dg;
         }
     }
-    double y[6];
     for (int i = 0; i < 6; i++) {
         double t = b[i];
         for (int k = 0; k < i; k++) t -= Lm[i * 6 + k] * y[k];
@@ -369,8 +381,8 @@ __device__ void normalize_mat4_safe(double* T) {
     T[1] = v[0]; T[5] = v[1]; T[9] = v[2];
     T[2] = w[0]; T[6] = w[1]; T[10] = w[2];
 }
-// Closed-form SE(3) exponential of generateA(x) (warp.cu:910-915).
-__device__ void se3_exp(const double* x, double* E) {
+// Closed-form SE(3) exponential of generateA(x) (warp.cu:910-915); w: 36 doubles.
+__device__ void se3_exp(const double* x, double* E, double* w) {
     double wx = x[0], wy = x[1], wz = x[2];
     double th2 = wx * wx + wy * wy + wz * wz, th = sqrt(th2);
     double a, b, cc;
@@ -382,15 +394,17 @@ __device__ void se3_exp(const double* x, double* E) {
         double sn = sin(th), co = cos(th);
         a = sn / th; b = (1.0 - co) / th2; cc = (th - sn) / (th2 * th);
     }
-    double Wm[9] = {0, -wz, wy, wz, 0, -wx, -wy, wx, 0};
-    double W2[9];
+    double* Wm = w;
+    double* W2 = w + 9;
+    double* Rm = w + 18;
+    double* V = w + 27;
+    Wm[0] = 0; Wm[1] = -wz; Wm[2] = wy; Wm[3] = wz; Wm[4] = 0; Wm[5] = -wx; Wm[6] = -wy; Wm[7] = wx; Wm[8] = 0;
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) {
             double s = 0;
             for (int k = 0; k < 3; k++) s += Wm[i * 3 + k] * Wm[k * 3 + j];
             W2[i * 3 + j] = s;
         }
-    double Rm[9], V[9];
     for (int i = 0; i < 9; i++) {
         double I = (i % 4 == 0) ? 1.0 : 0.0;
         Rm[i] = I + a * Wm[i] + b * W2[i];
@@ -402,14 +416,17 @@ __device__ void se3_exp(const double* x, double* E) {
     }
     E[12] = 0; E[13] = 0; E[14] = 0; E[15] = 1;
 }
-// expmKernel (expmCuda.cu:1989-2048): T <- normalize(T * normalize(exp(A(x))) with t scaled)
-__device__ void expm_update(const double* x, float* TF, double scale) {
-    double T[16], Ti[16], R[16];
+// expmKernel (expmCuda.cu:1989-2048): T <- normalize(T * normalize(exp(A(x))) with t scaled);
+// w: 84 doubles.
+__device__ void expm_update(const double* x, float* TF, double scale, double* w) {
+    double* T = w;
+    double* Ti = w + 16;
+    double* R = w + 32;
     for (int i = 0; i < 16; i++) T[i] = TF[i];
     double mx = 0;
     for (int i = 0; i < 6; i++) mx = fmax(mx, fabs(x[i]));
     if (mx != 0) {
-        se3_exp(x, Ti);
+        se3_exp(x, Ti, w + 48);
         normalize_mat4_safe(Ti);
         Ti[3] *= scale; Ti[7] *= scale; Ti[11] *= scale;
     } else {
@@ -428,20 +445,26 @@ __device__ void unpack27(const double* s, double* A, double* b) {
         for (int c2 = r; c2 < 6; c2++) { A[r * 6 + c2] = s[k]; A[c2 * 6 + r] = s[k]; k++; }
     for (int r = 0; r < 6; r++) b[r] = s[21 + r];
 }
-// estimateIncrement (phototracker.cpp:646-664) with linearFuse (warp.cu:1498-1509)
-__device__ void solve_update(const double* sys, float* trel, const SolveCfg& cfg, int N1, int N2) {
-    double A[36], b[6], x[6];
+// estimateIncrement (phototracker.cpp:646-664) with linearFuse (warp.cu:1498-1509);
+// w: SOLVE_WS doubles. Returns false on a degenerate system (Cholesky).
+__device__ bool solve_update(const double* sys, float* trel, const SolveCfg& cfg, int N1, int N2, double* w) {
+    double* A = w + WS_A;
+    double* b = w + WS_B;
+    double* x = w + WS_X;
     unpack27(cfg.mode == 0 ? sys + 27 : sys, A, b);
     if (cfg.mode == 2) {
-        double Ag[36], bg[6];
+        double* Ag = w + WS_AG;
+        double* bg = w + WS_BG;
         unpack27(sys + 27, Ag, bg);
         double s1 = (1.0 - (double)cfg.lambda) / (double)N1, s2 = (double)cfg.lambda / (double)N2;
         for (int k = 0; k < 36; k++) A[k] = s1 * A[k] + s2 * Ag[k];
         for (int k = 0; k < 6; k++) b[k] = s1 * b[k] + s2 * bg[k];
     }
-    if (cfg.solver == 1) solve_cg(A, b, x);
-    else solve_cholesky(A, b, x);
-    expm_update(x, trel, (double)cfg.scale_out);
+    bool ok = true;
+    if (cfg.solver == 1) solve_cg(A, b, x, w + WS_SOLVE);
+    else ok = solve_cholesky(A, b, x, w + WS_SOLVE);
+    expm_update(x, trel, (double)cfg.scale_out, w + WS_EXPM);
+    return ok;
 }
 
 
