@@ -381,12 +381,28 @@ __global__ __launch_bounds__(256) void k_inputs(const FrameSrc* __restrict__ src
         unsigned gs = 0;
         float ids = 0.0f;
         int nv = 0;
+        const uint8_t* g8 = s.g8[f];
+        const uint16_t* d16 = s.d16[f];
         for (int yy = 0; yy < bh; yy++) {
             size_t row = (size_t)(cy * bh + yy) * s.W + (size_t)cx * bw;
-            for (int xx = 0; xx < bw; xx++) {
-                gs += s.g8[f][row + xx];
-                unsigned d = s.d16[f][row + xx];
-                if (d) { ids += 1000.0f / (float)d; nv++; }
+            if (bw == 8 && !(((uintptr_t)(g8 + row)) & 7) && !(((uintptr_t)(d16 + row)) & 15)) {
+                // one 8-byte gray and one 16-byte depth load per row, same order of sums
+                uint2 gq = *(const uint2*)(g8 + row);
+                uint4 dq = *(const uint4*)(d16 + row);
+                const unsigned gw[2] = {gq.x, gq.y};
+                const unsigned dw[4] = {dq.x, dq.y, dq.z, dq.w};
+#pragma unroll
+                for (int xx = 0; xx < 8; xx++) {
+                    gs += (gw[xx >> 2] >> (8 * (xx & 3))) & 0xffu;
+                    unsigned d = (dw[xx >> 1] >> (16 * (xx & 1))) & 0xffffu;
+                    if (d) { ids += 1000.0f / (float)d; nv++; }
+                }
+            } else {
+                for (int xx = 0; xx < bw; xx++) {
+                    gs += g8[row + xx];
+                    unsigned d = d16[row + xx];
+                    if (d) { ids += 1000.0f / (float)d; nv++; }
+                }
             }
         }
         feat[2 * f] = (float)gs / (255.0f * (float)(bw * bh)) - 0.5f;

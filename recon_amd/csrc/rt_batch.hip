@@ -67,32 +67,33 @@ __global__ __launch_bounds__(256) void kb_pyramid(const PairDesc* __restrict__ P
 }
 
 // xyz2CloudKernel's gradient magnitude (d2z.cu:235-331) as an 8-bit bin per
-// pixel, and the 256-bin histogram (cuda_hist.cu:100-152). Tiles of 64x4: one
-// wave per image row segment; a wave adds each distinct bin it holds once.
+// pixel, and the 256-bin histogram (cuda_hist.cu:100-152). Tiles of 64 x 16
+// (one wave per 4 rows): the tile's histogram is counted in LDS (one LDS
+// atomic per pixel) and merged into the pair's with one global atomic per
+// non-empty bin.
+constexpr int BINS_TILE_H = 16;
 __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.z];
-    if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 4 >= d.H) return;
+    if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * BINS_TILE_H >= d.H) return;
     __shared__ float lut[256];
     __shared__ unsigned h[256];
     load_lut(lut);
     h[threadIdx.x] = 0u;
     __syncthreads();
-    int lane = lane_id();
-    int x = blockIdx.x * 64 + lane, y = blockIdx.y * 4 + (threadIdx.x >> 6);
-    bool in = x < d.W && y < d.H;
-    unsigned b = 0;
-    if (in) {
-        PixFeat f = pixel_features(U16DepthSrc{d.d16[0]}, U8Src{d.g8[0], lut}, d.W, d.H, x, y, d.calib);
-        b = grad_bin(f.gmag);
-        d.bins[(size_t)x + (size_t)y * d.W] = (uint8_t)b;
-    }
-    uint64_t active = __ballot(in);
-    while (active) {
-        int leader = __ffsll((unsigned long long)active) - 1;
-        unsigned bl = __shfl(b, leader, 64);
-        uint64_t m = __ballot(in && b == bl);
-        if (lane == leader) atomicAdd(&h[bl], (unsigned)__popcll(m));
-        active &= ~m;
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;
+    const U16DepthSrc zs{d.d16[0]};
+    const U8Src gs{d.g8[0], lut};
+    RT_GLOBAL uint8_t* bins = gmem(d.bins);
+#pragma unroll
+    for (int k = 0; k < BINS_TILE_H / 4; k++) {
+        const int y = blockIdx.y * BINS_TILE_H + wave * (BINS_TILE_H / 4) + k;
+        if (x < d.W && y < d.H) {
+            PixFeat f = pixel_features(zs, gs, d.W, d.H, x, y, d.calib);
+            unsigned b = grad_bin(f.gmag);
+            bins[(size_t)x + (size_t)y * d.W] = (uint8_t)b;
+            atomicAdd(&h[b], 1u);
+        }
     }
     __syncthreads();
     unsigned v = h[threadIdx.x];
@@ -456,7 +457,7 @@ static void launch_prep(rt_batch* b, hipStream_t s) {
     dim3 g64((b->maxW + 63) / 64, (b->maxH + 63) / 64, n);
     kb_pyramid<<<g64, 256, 0, s>>>(b->desc_dev, 0);
     kb_pyramid<<<g64, 256, 0, s>>>(b->desc_dev, 1);
-    dim3 gb((b->maxW + 63) / 64, (b->maxH + 3) / 4, n);
+    dim3 gb((b->maxW + 63) / 64, (b->maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
     kb_bins<<<gb, 256, 0, s>>>(b->desc_dev);
     kb_select_count<<<dim3(128, n), 256, 0, s>>>(b->desc_dev);
     kb_select_scan<<<n, 512, 0, s>>>(b->desc_dev);
