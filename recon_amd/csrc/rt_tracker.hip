@@ -5,6 +5,7 @@
 #include <math.h>
 #include <stdio.h>
 #include <string.h>
+#include <algorithm>
 #include <string>
 #include <vector>
 #include "../../include/rt_tracker.h"
@@ -571,8 +572,13 @@ extern "C" int rt_track_sequence(rt_tracker* t, const uint8_t* gray, const uint1
     if (fuse_frames < 0 || fuse_frames >= interval) return set_error(RT_ERR_ARG, "fuse_frames must be in [0, interval)");
     HIP_TRY(hipSetDevice(t->device));
     if (t->seq_cap < n) {
-        if (t->seq_poses) hipFree(t->seq_poses);
+        if (t->seq_poses) {
+            // drop it from the allocation list too: destroy must not free it again
+            t->allocs.erase(std::remove(t->allocs.begin(), t->allocs.end(), (void*)t->seq_poses), t->allocs.end());
+            HIP_TRY(hipFree(t->seq_poses));
+        }
         t->seq_poses = nullptr;
+        t->seq_cap = 0;
         HIP_TRY(hipMalloc((void**)&t->seq_poses, (size_t)n * 16 * sizeof(double)));
         t->allocs.push_back(t->seq_poses);
         t->seq_cap = n;

@@ -146,3 +146,21 @@ def test_bi_objective_sequence_300_frames_640x480():
     print(f"300 frames in {dt:.2f} s ({n / dt:.0f} frames/s); final error {et[-1, 0]:.3f} deg {et[-1, 1]:.2f} mm; "
           f"max {et[:, 0].max():.3f} deg {et[:, 1].max():.2f} mm", flush=True)
     assert et[:, 0].max() < 2.0 and et[:, 1].max() < 60.0, et.max(axis=0)
+
+
+def test_sequence_driver_grows_its_pose_buffer():
+    """A second, longer sequence on the same tracker reallocates the per-frame
+    pose buffer; the poses equal those of a fresh tracker and the tracker is
+    destroyed cleanly afterwards."""
+    W, H = 160, 120
+    print("case sequence regrow", flush=True)
+    g, d, gt = render_sequence(41, 12, W, H, 1.0, 15.0)
+    cal = synth.calib_block_np(W, H)
+    a = PhotoTracker(W, H, cal, mode=PHOTOMETRIC, n_select=2048)
+    a.track_sequence(g.ptr, d.ptr, 5, 4, 2)
+    Pa = a.track_sequence(g.ptr, d.ptr, 12, 4, 2)
+    a.close()
+    b = PhotoTracker(W, H, cal, mode=PHOTOMETRIC, n_select=2048)
+    Pb = b.track_sequence(g.ptr, d.ptr, 12, 4, 2)
+    b.close()
+    assert np.array_equal(Pa, Pb)
