@@ -168,7 +168,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--extras", action="store_true",
-                    help="also run the single-pair, sequence and mixed-batch cases (off: see DESIGN.md, known GPU fault)")
+                    help="also run the single-pair, sequence and mixed-batch cases (off by default: this leg once "
+                         "faulted the GPU in its mixed-batch part, cause not yet found)")
     ap.add_argument("--track-only", action="store_true",
                     help="prepare the batch and run --steps tracking launches only (for counter collection)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
