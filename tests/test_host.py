@@ -16,10 +16,12 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_library_exports_every_declared_symbol():
-    hdr = open(os.path.join(ROOT, "include", "rt_tracker.h")).read()
-    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
-    names = set(re.findall(r"\b(rt_[a-z0-9_]+)\s*\(", hdr))
-    assert len(names) >= 40
+    names = set()
+    for h in ("rt_tracker.h", "rt_cnn.h"):
+        hdr = open(os.path.join(ROOT, "include", h)).read()
+        hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+        names |= set(re.findall(r"\b(rt_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) >= 55
     L = lib()
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
