@@ -200,6 +200,10 @@ int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_result* out
  * rt_batch_propose). out NULL: enqueue without waiting. */
 int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_result* out);
 void* rt_batch_stream(rt_batch* b);
+/* Inspection of one pair of the last prepared batch (for parity tests):
+ * RT_DBG_GRAD_HIST, RT_DBG_SELECTED, RT_DBG_POINTS, RT_DBG_JACOBIAN (level),
+ * RT_DBG_TREL (the pair's initial pose); layouts as for single trackers. */
+int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, void* host, size_t bytes);
 
 /* ------------------------------------------------------------------ inspection
  * Read an internal device buffer into host memory, for parity tests.

@@ -75,11 +75,22 @@ constexpr int BINS_TILE_H = 16;
 __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.z];
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * BINS_TILE_H >= d.H) return;
+    constexpr int TW = 64 + 4, TH = BINS_TILE_H + 4;  // tile plus a 2-pixel halo
     __shared__ float lut[256];
     __shared__ unsigned h[256];
+    __shared__ float gt[TW * TH];
     load_lut(lut);
     h[threadIdx.x] = 0u;
     __syncthreads();
+    const int W = d.W, H = d.H;
+    const int tx0 = (int)blockIdx.x * 64 - 2, ty0 = (int)blockIdx.y * BINS_TILE_H - 2;
+    const RT_GLOBAL uint8_t* g8 = gmem(d.g8[0]);
+    for (int e = threadIdx.x; e < TW * TH; e += 256) {
+        int gx = tx0 + e % TW, gy = ty0 + e / TW;
+        gt[e] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? lut[g8[(size_t)gx + (size_t)gy * W]] : 0.0f;
+    }
+    __syncthreads();
+    const LdsTile tile{gt, tx0, ty0, TW, TH};
     const int lane = lane_id(), wave = threadIdx.x >> 6;
     const int x = blockIdx.x * 64 + lane;
     const U16DepthSrc zs{d.d16[0]};
@@ -88,10 +99,10 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
 #pragma unroll
     for (int k = 0; k < BINS_TILE_H / 4; k++) {
         const int y = blockIdx.y * BINS_TILE_H + wave * (BINS_TILE_H / 4) + k;
-        if (x < d.W && y < d.H) {
-            PixFeat f = pixel_features(zs, gs, d.W, d.H, x, y, d.calib);
+        if (x < W && y < H) {
+            PixFeat f = pixel_features_tiled(zs, gs, tile, W, H, x, y, d.calib);
             unsigned b = grad_bin(f.gmag);
-            bins[(size_t)x + (size_t)y * d.W] = (uint8_t)b;
+            bins[(size_t)x + (size_t)y * W] = (uint8_t)b;
             atomicAdd(&h[b], 1u);
         }
     }
@@ -680,6 +691,32 @@ extern "C" int rt_batch_propose(rt_batch* b, rt_cnn* net) {
         src[i].d16[0] = d.d16[0]; src[i].d16[1] = d.d16[1];
     }
     return cnn_propose(net, b->n, src.data(), b->trel_in, b->stream);
+}
+
+// Inspection of one pair of the last prepared batch: RT_DBG_GRAD_HIST,
+// RT_DBG_SELECTED, RT_DBG_POINTS, RT_DBG_JACOBIAN (level), RT_DBG_TREL (initial
+// pose of the pair).
+extern "C" int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, void* host, size_t bytes) {
+    if (!b || !host || pair < 0 || pair >= b->n) return set_error(RT_ERR_ARG, "rt_batch_debug_read: bad argument");
+    if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    const PairDesc& d = b->desc[pair];
+    const void* src = nullptr;
+    size_t n = 0;
+    switch (what) {
+        case RT_DBG_GRAD_HIST: src = d.ghist; n = 256 * 4; break;
+        case RT_DBG_SELECTED: src = d.sel; n = (size_t)d.N * 4; break;
+        case RT_DBG_POINTS: src = d.pts; n = (size_t)d.N * 16; break;
+        case RT_DBG_JACOBIAN:
+            if (level < 0 || level >= d.L) return set_error(RT_ERR_ARG, "bad level");
+            src = d.jac[level]; n = jac_bytes(d.N); break;
+        case RT_DBG_TREL: src = d.trel_in; n = 64; break;
+        default: return set_error(RT_ERR_UNSUPPORTED, "rt_batch_debug_read: not available for batches");
+    }
+    if (bytes < n) return set_error(RT_ERR_ARG, "buffer too small");
+    if (hipMemcpyAsync(host, src, n, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipStreamSynchronize(b->stream) != hipSuccess)
+        return set_error(RT_ERR_HIP, "rt_batch_debug_read: copy");
+    return RT_OK;
 }
 
 // out == NULL: enqueue the launch and return without waiting (timing).

@@ -186,6 +186,60 @@ __device__ __forceinline__ PixFeat pixel_features(const float* __restrict__ z, c
     return pixel_features(F32Src{z}, F32Src{g0}, W, H, xi, yi, c);
 }
 
+// A block of gray values staged in LDS: image pixels [x0, x0+w) x [y0, y0+h),
+// row pitch w, holding exactly the values the image source returns.
+struct LdsTile {
+    const float* p;
+    int x0, y0, w, h;
+    __device__ __forceinline__ bool covers(int xa, int ya, int xb, int yb) const {
+        return xa >= x0 && ya >= y0 && xb < x0 + w && yb < y0 + h;
+    }
+};
+// pixel_features with the 4x4 tap neighbourhood read from an LDS tile when it
+// lies inside it (the same values, so the same results), else from the image.
+template <class ZS, class GS>
+__device__ __forceinline__ PixFeat pixel_features_tiled(const ZS& z, const GS& g0, const LdsTile& tile, int W, int H,
+                                                        int xi, int yi, const float* c) {
+    PixFeat f;
+    f3 p3;
+    backproject(z[(size_t)xi + (size_t)yi * W], xi, yi, c, p3);
+    if (p3.z < 0.0f) {
+        f3 r3 = mul4(c + TLR, p3);
+        f2 pu{r3.x / r3.z, r3.y / r3.z};
+        f2 p = distort(pu, c + KCR, c + KR);
+        if (in_bounds(p, W, H)) {
+            int xdi = (int)p.x, ydi = (int)p.y;
+            float fx = p.x - (float)xdi, fy = p.y - (float)ydi;
+            int xe = (int)(p.x + 1.0f), xw = (int)(p.x - 1.0f), yn = (int)(p.y - 1.0f), ys = (int)(p.y + 1.0f);
+            if (xe == xdi + 1 && xw == xdi - 1 && yn == ydi - 1 && ys == ydi + 1 &&
+                tile.covers(xdi - 1, ydi - 1, xdi + 2, ydi + 2)) {
+                const float* q = tile.p + (ydi - tile.y0) * tile.w + (xdi - tile.x0);
+                const int P = tile.w;
+                float a01 = q[-P], a02 = q[-P + 1];
+                float a10 = q[-1], a11 = q[0], a12 = q[1], a13 = q[2];
+                float a20 = q[P - 1], a21 = q[P], a22 = q[P + 1], a23 = q[P + 2];
+                float a31 = q[2 * P], a32 = q[2 * P + 1];
+                float c1 = blend(a11, a12, a21, a22, fx, fy);
+                float cN = blend(a01, a02, a11, a12, fx, fy);
+                float cE = blend(a12, a13, a22, a23, fx, fy);
+                float cW = blend(a10, a11, a20, a21, fx, fy);
+                float cS = blend(a21, a22, a31, a32, fx, fy);
+                f.gx = (cE - cW) / 2.0f;
+                f.gy = (cS - cN) / 2.0f;
+                f.gmag = fminf(fabsf(f.gy) + fabsf(f.gx), 1.0f);
+                f.c1 = c1;
+                f.p = p3;
+                f.ok = true;
+                return f;
+            }
+            return pixel_features(z, g0, W, H, xi, yi, c);
+        }
+    }
+    f.p = f3{0.0f, 0.0f, -c[MAXD]};
+    f.c1 = 0.0f; f.gx = 0.0f; f.gy = 0.0f; f.gmag = 0.0f; f.ok = false;
+    return f;
+}
+
 // gradient-magnitude bin (gpu_hist.cu:135, filterIndex256.cu:126), fused multiply-add
 __device__ __forceinline__ unsigned grad_bin(float g) { return (unsigned)__builtin_fmaf(g, 255.0f, 0.5f); }
 

@@ -268,6 +268,12 @@ class BatchTracker:
         arr, n, _out, _keep = prepared
         check(self.L.rt_batch_prepare(self.h, n, arr), "rt_batch_prepare")
 
+    def debug_read(self, pair, what, level, dtype, count):
+        out = np.zeros(count, dtype)
+        check(self.L.rt_batch_debug_read(self.h, pair, DBG[what], level, out.ctypes.data, out.nbytes),
+              "rt_batch_debug_read")
+        return out
+
     def propose(self, net):
         """Initial poses of the prepared pairs from the pose network (PoseNet)."""
         check(self.L.rt_batch_propose(self.h, net.h), "rt_batch_propose")

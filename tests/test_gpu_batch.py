@@ -69,3 +69,34 @@ def test_batch_retrack_deterministic_and_warm_start():
         assert rot < 0.15 and tmm < 3.0
         checked += 1
     assert checked >= 2
+
+
+@pytest.mark.parametrize("N", [2048])
+def test_batch_preparation_bitexact_against_oracle(N):
+    """The batch path's keyframe preparation (gradient bins from LDS-staged
+    tiles, histogram, selection, points, Jacobians) equals the FP32 oracle bit
+    for bit, pair by pair, on a mixed-size batch."""
+    print(f"case batch preparation N={N}", flush=True)
+    from tests._oracle_steps import Steps
+    n = 4
+    pairs, host, gts = _batch(n, seed=60)
+    bt = BatchTracker(n, 640, 480, mode=PHOTOMETRIC, n_select=N)
+    bt.prepare_only(bt.prepare(pairs))
+    for i in range(n):
+        W, H = pairs[i]["width"], pairs[i]["height"]
+        L = 4 if W == 640 else 3
+        g0, d0, g1, d1, _ = host[i]
+        st = Steps(W, H, L, pairs[i]["calib"], g0, d0, g1, d1, nselect=N, dtype=np.float32)
+        st.features()
+        assert np.array_equal(bt.debug_read(i, "GRAD_HIST", 0, np.uint32, 256), st.hist), f"histogram pair {i}"
+        assert np.array_equal(bt.debug_read(i, "SELECTED", 0, np.int32, N), st.sel), f"selection pair {i}"
+        pts = bt.debug_read(i, "POINTS", 0, np.float32, 4 * N).reshape(N, 4)
+        rec = st.rec.reshape(-1, 24)[st.sel]
+        assert np.array_equal(pts[:, :3], rec[:, :3]), f"points pair {i}"
+        color_slot = [14, 17, 20, 23]
+        for l in range(L):
+            jac = bt.debug_read(i, "JACOBIAN", l, np.float32, 7 * N)
+            J = st.J[l].reshape(6, N).T
+            got = np.concatenate([jac[:4 * N].reshape(N, 4), jac[4 * N:6 * N].reshape(N, 2)], axis=1)
+            assert np.array_equal(got, J), f"jacobian pair {i} level {l}"
+            assert np.array_equal(jac[6 * N:], rec[:, color_slot[l]]), f"reference gray pair {i} level {l}"
