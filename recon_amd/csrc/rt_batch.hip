@@ -111,32 +111,75 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     if (v) atomicAdd(&d.ghist[threadIdx.x], v);
 }
 
-// filterIndexKernel4 (filterIndex256.cu:110-169) pass 1 with one wave per
-// segment: counts of bins above and equal to the threshold.
-__global__ __launch_bounds__(256) void kb_select_count(const PairDesc* __restrict__ P) {
-    const PairDesc& d = P[blockIdx.y];
+// seekThreshold256Kernel2 (filterIndex256.cu:26-46): threshold bin and excess,
+// once per pair.
+__global__ __launch_bounds__(256) void kb_select_thresh(const PairDesc* __restrict__ P) {
+    const PairDesc& d = P[blockIdx.x];
     int thr, excess;
     find_threshold(d.ghist, d.N, thr, excess);
+    if (threadIdx.x == 0) { d.seg[2560] = thr; d.seg[2561] = excess; }
+}
+
+// Four bins of one lane (pixels base .. base+3 of a segment), as above/equal flags.
+__device__ __forceinline__ void lane_bins4(const RT_GLOBAL uint8_t* bins, int base, int end, bool vec, unsigned thr,
+                                           unsigned& a4, unsigned& e4) {
+    a4 = 0u; e4 = 0u;
+    unsigned w = 0u;
+    if (vec) {
+        if (base < end) w = *(const RT_GLOBAL unsigned*)(bins + base);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (base + j < end) w |= (unsigned)bins[base + j] << (8 * j);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        if (base + j >= end) break;
+        unsigned bn = (w >> (8 * j)) & 0xffu;
+        a4 |= (unsigned)(bn > thr) << j;
+        e4 |= (unsigned)(bn == thr) << j;
+    }
+}
+// inclusive sum over the lanes of a wave
+__device__ __forceinline__ int wave_scan_incl(int v) {
+    const int lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(v, o, 64);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+// filterIndexKernel4 (filterIndex256.cu:110-169) pass 1 with one wave per
+// segment of npix/512 pixels: counts of bins above and equal to the threshold.
+// Each lane reads four bins at a time.
+__global__ __launch_bounds__(256) void kb_select_count(const PairDesc* __restrict__ P) {
+    const PairDesc& d = P[blockIdx.y];
+    const unsigned thr = (unsigned)d.seg[2560];
     int wave = threadIdx.x >> 6, lane = lane_id();
     int s = blockIdx.x * 4 + wave;
-    int npix = d.W * d.H, Pn = npix / 512, b0 = s * Pn;
+    int npix = d.W * d.H, Pn = npix / 512, b0 = s * Pn, end = b0 + Pn;
+    const bool vec = (Pn & 3) == 0;
+    const RT_GLOBAL uint8_t* bins = gmem(d.bins);
     int cA = 0, cE = 0;
-    for (int off = 0; off < Pn; off += 64) {
-        bool in = off + lane < Pn;
-        unsigned bn = in ? d.bins[b0 + off + lane] : 0u;
-        cA += __popcll(__ballot(in && bn > (unsigned)thr));
-        cE += __popcll(__ballot(in && bn == (unsigned)thr));
+    for (int off = 0; off < Pn; off += 256) {
+        unsigned a4, e4;
+        lane_bins4(bins, b0 + off + 4 * lane, end, vec, thr, a4, e4);
+        cA += __popc(a4);
+        cE += __popc(e4);
     }
+    cA = (int)wave_sum((float)cA);
+    cE = (int)wave_sum((float)cE);
     if (lane == 0) {
         d.seg[s] = cA;
         d.seg[512 + s] = cE;
-        if (s == 0) { d.seg[2560] = thr; d.seg[2561] = excess; }
     }
 }
 __global__ __launch_bounds__(512) void kb_select_scan(const PairDesc* __restrict__ P) {
     select_scan_block(P[blockIdx.x].seg);
 }
-// pass 3: ordered scatter, one wave per segment
+// pass 3: ordered scatter, one wave per segment, four bins per lane
 __global__ __launch_bounds__(256) void kb_select_scatter(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.y];
     int wave = threadIdx.x >> 6, lane = lane_id();
@@ -144,24 +187,32 @@ __global__ __launch_bounds__(256) void kb_select_scatter(const PairDesc* __restr
     const int* seg = d.seg;
     unsigned thr = (unsigned)seg[2560];
     int startA = seg[1024 + s], startE = seg[1536 + s], mE = seg[2048 + s];
-    int npix = d.W * d.H, Pn = npix / 512, b0 = s * Pn, N = d.N;
-    uint64_t lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+    int npix = d.W * d.H, Pn = npix / 512, b0 = s * Pn, end = b0 + Pn, N = d.N;
+    const bool vec = (Pn & 3) == 0;
+    const RT_GLOBAL uint8_t* bins = gmem(d.bins);
+    RT_GLOBAL int* sel = gmem(d.sel);
     int rA = 0, rE = 0;
-    for (int off = 0; off < Pn; off += 64) {
-        int i = b0 + off + lane;
-        bool in = off + lane < Pn;
-        unsigned bn = in ? d.bins[i] : 0u;
-        bool isA = in && bn > thr, isE = in && bn == thr;
-        uint64_t ma = __ballot(isA), me = __ballot(isE);
-        if (isA) { int pos = startA + rA + __popcll(ma & lt); if (pos < N) d.sel[pos] = i; }
-        int e = rE + __popcll(me & lt);
-        if (isE && e < mE) { int pos = startE + e; if (pos < N) d.sel[pos] = i; }
-        rA += __popcll(ma);
-        rE += __popcll(me);
+    for (int off = 0; off < Pn; off += 256) {
+        const int base = b0 + off + 4 * lane;
+        unsigned a4, e4;
+        lane_bins4(bins, base, end, vec, thr, a4, e4);
+        int na = __popc(a4), ne = __popc(e4);
+        int ia = wave_scan_incl(na), ie = wave_scan_incl(ne);
+        int pa = rA + ia - na, pe = rE + ie - ne;  // ranks of this lane's first hits
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            if ((a4 >> j) & 1u) { int pos = startA + pa; if (pos < N) sel[pos] = base + j; pa++; }
+            if ((e4 >> j) & 1u) {
+                if (pe < mE) { int pos = startE + pe; if (pos < N) sel[pos] = base + j; }
+                pe++;
+            }
+        }
+        rA += __shfl(ia, 63, 64);
+        rE += __shfl(ie, 63, 64);
     }
     if (s == 0) {
         int total = seg[2563];
-        for (int k = (total > 0 ? total : 0) + lane; k < N; k += 64) d.sel[k] = 0;
+        for (int k = (total > 0 ? total : 0) + lane; k < N; k += 64) sel[k] = 0;
     }
 }
 __global__ __launch_bounds__(256) void kb_ref_points(const PairDesc* __restrict__ P) {
@@ -470,6 +521,7 @@ static void launch_prep(rt_batch* b, hipStream_t s) {
     kb_pyramid<<<g64, 256, 0, s>>>(b->desc_dev, 1);
     dim3 gb((b->maxW + 63) / 64, (b->maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
     kb_bins<<<gb, 256, 0, s>>>(b->desc_dev);
+    kb_select_thresh<<<n, 256, 0, s>>>(b->desc_dev);
     kb_select_count<<<dim3(128, n), 256, 0, s>>>(b->desc_dev);
     kb_select_scan<<<n, 512, 0, s>>>(b->desc_dev);
     kb_select_scatter<<<dim3(128, n), 256, 0, s>>>(b->desc_dev);
