@@ -450,7 +450,7 @@ struct rt_batch {
     std::map<std::string, std::pair<int, int>> tuned;  // key -> (variant, grid)
     int variant = -1, grid = 0;
     hipGraphExec_t graph_prep = nullptr, graph_track = nullptr;
-    int graph_n = -1, graph_variant = -1, graph_grid = -1;
+    int graph_n = -1, graph_variant = -1, graph_grid = -1, graph_maxN = -1;  // the track graph's launch shape
 };
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
@@ -649,9 +649,12 @@ static int fetch_results(rt_batch* b, rt_pair_result* out) {
 static int run_track(rt_batch* b, rt_pair_result* out) {
     int rc = autotune(b);
     if (rc) return rc;
-    if (!b->graph_track || b->graph_n != b->n || b->graph_variant != b->variant || b->graph_grid != b->grid) {
+    // the captured launch depends on the batch size, the variant, the grid and,
+    // through its dynamic LDS, on the largest point count
+    if (!b->graph_track || b->graph_n != b->n || b->graph_variant != b->variant || b->graph_grid != b->grid ||
+        b->graph_maxN != b->maxN) {
         if ((rc = capture(b, &b->graph_track, false))) return rc;
-        b->graph_n = b->n; b->graph_variant = b->variant; b->graph_grid = b->grid;
+        b->graph_n = b->n; b->graph_variant = b->variant; b->graph_grid = b->grid; b->graph_maxN = b->maxN;
     }
     if (hipGraphLaunch(b->graph_track, b->stream) != hipSuccess) return set_error(RT_ERR_HIP, "graph launch");
     return out ? fetch_results(b, out) : RT_OK;
