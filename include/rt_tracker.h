@@ -116,6 +116,16 @@ int rt_set_reference_rgbd(rt_tracker* t, const float* rgbd, int on_device, int s
  * ready (no keyframe or no current frame for the mode), < 0 on error. */
 int rt_track(rt_tracker* t);
 
+/* rt_track with the Gauss-Newton loop started from T_init, the current camera
+ * pose relative to the keyframe (row-major 4x4, mm), instead of the previous
+ * estimate. No counterpart in the reference: the warm start of a learned pose
+ * proposal. Returns as rt_track. */
+int rt_track_from(rt_tracker* t, const double T_init[16]);
+/* Weighted photometric residual RMS of the last iteration, sqrt(sum (w r)^2 /
+ * sum w^2) over the selected points (photometric and bi-objective modes). No
+ * counterpart in the reference: the trigger of relocalisation. */
+int rt_get_residual_rms(rt_tracker* t, float* rms);
+
 /* getGlobalPose / getRelativePose / getIncrement (phototracker.cpp:223-229, 564) */
 int rt_get_global_pose(const rt_tracker* t, double T[16]);
 int rt_get_relative_pose(const rt_tracker* t, float T[16]);

@@ -54,6 +54,8 @@ SIGNATURES = {
     "rt_set_reference_rgbd": (c_int, [c_void_p, c_void_p, c_int, c_int, c_float, c_int]),
     "rt_track": (c_int, [c_void_p]),
     "rt_get_global_pose": (c_int, [c_void_p, P(c_double)]),
+    "rt_track_from": (c_int, [c_void_p, P(c_double)]),
+    "rt_get_residual_rms": (c_int, [c_void_p, P(c_float)]),
     "rt_get_relative_pose": (c_int, [c_void_p, P(c_float)]),
     "rt_get_increment": (c_int, [c_void_p, P(c_double)]),
     "rt_reset_tracking": (c_int, [c_void_p, c_int]),

@@ -401,6 +401,48 @@ extern "C" int rt_track(rt_tracker* t) {
     return 1;
 }
 
+// rt_track with the Gauss-Newton loop started from a given pose of the current
+// camera relative to the keyframe (a warm start, e.g. a learned proposal)
+// instead of the previous estimate. The pose bookkeeping (applyIncrement) is
+// that of rt_track.
+extern "C" int rt_track_from(rt_tracker* t, const double* T_init) {
+    if (!t || !T_init) return set_error(RT_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(t->device));
+    if (!ready(t)) return 0;
+    int rc = preprocess(t, false);
+    if (rc) return rc;
+    float Tf[16], iT[16];
+    for (int i = 0; i < 16; i++) Tf[i] = (float)T_init[i];
+    invert_rt4f(Tf, iT);
+    memcpy(t->trel_host, iT, sizeof(iT));  // pinned staging, read by the copy before anything rewrites it
+    HIP_TRY(hipMemcpyAsync(t->d.trel, t->trel_host, 16 * sizeof(float), hipMemcpyHostToDevice, t->stream));
+    if ((rc = estimate(t))) return rc;
+    HIP_TRY(hipMemcpyAsync(t->trel_host, t->d.trel, 16 * sizeof(float), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    apply_increment(t->trel_host, t->T, t->Tinc, t->global);
+    return 1;
+}
+
+// Weighted photometric residual RMS of the last Gauss-Newton iteration,
+// sqrt(sum (w r)^2 / sum w^2) over the selected points (0 when all weights vanish).
+extern "C" int rt_get_residual_rms(rt_tracker* t, float* rms) {
+    if (!t || !rms) return set_error(RT_ERR_ARG, "null argument");
+    if (t->mode == RT_GEOMETRIC) return set_error(RT_ERR_STATE, "no photometric residuals in geometric mode");
+    HIP_TRY(hipSetDevice(t->device));
+    std::vector<float> rz(2 * (size_t)t->N), w(t->N);
+    HIP_TRY(hipMemcpyAsync(rz.data(), t->d.res, rz.size() * 4, hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipMemcpyAsync(w.data(), t->d.wts, w.size() * 4, hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    double a = 0.0, b = 0.0;
+    for (int i = 0; i < t->N; i++) {
+        double wr = (double)w[i] * rz[2 * (size_t)i];
+        a += wr * wr;
+        b += (double)w[i] * w[i];
+    }
+    *rms = b > 0.0 ? (float)sqrt(a / b) : 0.0f;
+    return RT_OK;
+}
+
 extern "C" int rt_get_global_pose(const rt_tracker* t, double* T) {
     if (!t || !T) return set_error(RT_ERR_ARG, "null argument");
     memcpy(T, t->global, sizeof(t->global));

@@ -88,6 +88,18 @@ class PhotoTracker:
     def track(self):
         return bool(check(self.L.rt_track(self.h), "rt_track"))
 
+    def track_from(self, T_init):
+        """Track the current frame starting from T_init (current camera relative
+        to the keyframe)."""
+        t = np.ascontiguousarray(T_init, np.float64).ravel()
+        return bool(check(self.L.rt_track_from(self.h, t.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                          "rt_track_from"))
+
+    def residual_rms(self):
+        r = ctypes.c_float()
+        check(self.L.rt_get_residual_rms(self.h, ctypes.byref(r)), "rt_get_residual_rms")
+        return float(r.value)
+
     # ---- poses
     def get_global_pose(self):
         T = np.zeros(16, np.float64)
