@@ -113,6 +113,95 @@ __global__ __launch_bounds__(256) void k_conv_fwd(ConvGeom g, const bf16* __rest
     }
 }
 
+// Forward with LDS tiles, for Cin a multiple of 32 and Cout a multiple of 64:
+// block tile 128 output pixels x 64 output channels, k in steps of 32 (one tap,
+// 32 consecutive input channels: 64 contiguous bytes per row). The im2col rows
+// and the weights are staged in LDS once per block (rows padded to 72 bytes so
+// that the 16 rows an MFMA operand reads hit distinct banks); each wave
+// computes 32 pixels x 64 channels (2 x 4 MFMA tiles).
+constexpr int FT_M = 128, FT_N = 64, FT_K = 32, FT_LD = FT_K + 4;  // LD in bf16 elements
+__global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvGeom g, const bf16* __restrict__ X,
+                                                      const bf16* __restrict__ Wp, const float* __restrict__ bias,
+                                                      bf16* __restrict__ Y) {
+    __shared__ __attribute__((aligned(16))) bf16 As[FT_M * FT_LD];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[FT_N * FT_LD];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int r16 = lane & 15, kq = lane >> 4;
+    const int M = g.B * g.Ho * g.Wo, HW = g.Ho * g.Wo;
+    const int row0 = blockIdx.x * FT_M, col0 = blockIdx.y * FT_N;
+    // this thread's two A rows (16-byte chunks: row = e / 4, chunk = e % 4)
+    int an[2], ayo[2], axo[2], arow[2], achunk[2];
+    bool av[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        int e = t + q * 256;
+        arow[q] = e >> 2;
+        achunk[q] = e & 3;
+        int m = row0 + arow[q];
+        av[q] = m < M;
+        int mm = av[q] ? m : 0;
+        an[q] = mm / HW;
+        int r = mm - an[q] * HW;
+        ayo[q] = r / g.Wo;
+        axo[q] = r - ayo[q] * g.Wo;
+    }
+    const int brow = t >> 2, bchunk = t & 3;
+    f4 acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < g.K; k0 += FT_K) {
+        const int tap = k0 >> g.ci_shift, ci0 = k0 & (g.Ci - 1);
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            int yi = 2 * ayo[q] - 1 + ky, xi = 2 * axo[q] - 1 + kx;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (av[q] && yi >= 0 && yi < g.Hi && xi >= 0 && xi < g.Wi)
+                v = *(const uint4*)(X + (((size_t)an[q] * g.Hi + yi) * g.Wi + xi) * g.Ci + ci0 + 8 * achunk[q]);
+            uint2* dst = (uint2*)(As + arow[q] * FT_LD + 8 * achunk[q]);
+            dst[0] = make_uint2(v.x, v.y);
+            dst[1] = make_uint2(v.z, v.w);
+        }
+        {
+            uint4 v = *(const uint4*)(Wp + (size_t)(col0 + brow) * g.Kp + k0 + 8 * bchunk);
+            uint2* dst = (uint2*)(Bs + brow * FT_LD + 8 * bchunk);
+            dst[0] = make_uint2(v.x, v.y);
+            dst[1] = make_uint2(v.z, v.w);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < FT_K; kk += 16) {
+            s4 a[2], b[4];
+#pragma unroll
+            for (int i = 0; i < 2; i++) a[i] = *(const s4*)(As + (w * 32 + i * 16 + r16) * FT_LD + kk + 4 * kq);
+#pragma unroll
+            for (int j = 0; j < 4; j++) b[j] = *(const s4*)(Bs + (j * 16 + r16) * FT_LD + kk + 4 * kq);
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++) acc[i][j] = mfma(a[i], b[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        int col = col0 + j * 16 + r16;
+        float bv = bias[col];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                int m = row0 + w * 32 + i * 16 + kq * 4 + e;
+                if (m < M) {
+                    float v = acc[i][j][e] + bv;
+                    Y[(size_t)m * g.Co + col] = f2bf(v > 0.0f ? v : 0.0f);
+                }
+            }
+    }
+}
+
 // ---------------------------------------------------------------- input gradient
 // dX = conv^T(dZ, W), masked by the ReLU of the layer input (X > 0), stored as
 // the next (earlier) layer's dZ. WT: [Cin][9][Cout].
@@ -657,8 +746,14 @@ static int forward(rt_cnn* net, int n, const bf16* x, hipStream_t s) {
     for (int l = 0; l < 6; l++) {
         ConvGeom g = geom(net, l, n);
         int M = n * g.Ho * g.Wo;
-        dim3 grid((M + 63) / 64, (g.Co + 63) / 64);
-        k_conv_fwd<<<grid, 256, 0, s>>>(g, net->act[l], net->Wp[l], net->params + net->b_off[l], net->act[l + 1]);
+        if (g.Ci % FT_K == 0 && g.Co % FT_N == 0) {
+            dim3 grid((M + FT_M - 1) / FT_M, g.Co / FT_N);
+            k_conv_fwd_lds<<<grid, 256, 0, s>>>(g, net->act[l], net->Wp[l], net->params + net->b_off[l],
+                                                net->act[l + 1]);
+        } else {
+            dim3 grid((M + 63) / 64, (g.Co + 63) / 64);
+            k_conv_fwd<<<grid, 256, 0, s>>>(g, net->act[l], net->Wp[l], net->params + net->b_off[l], net->act[l + 1]);
+        }
     }
     return hipGetLastError() == hipSuccess ? RT_OK : set_error(RT_ERR_HIP, "rt_cnn forward launch");
 }
