@@ -39,7 +39,6 @@ struct PairDesc {
     float* z[2];
     uint8_t* bins;          // per-pixel gradient-magnitude bin
     unsigned* ghist;
-    int* seg;
     int* sel;
     float4* pts;
     float4* jac[4];
@@ -75,32 +74,70 @@ constexpr int BINS_TILE_H = 16;
 __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.z];
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * BINS_TILE_H >= d.H) return;
-    constexpr int TW = 64 + 4, TH = BINS_TILE_H + 4;  // tile plus a 2-pixel halo
+    // tile plus a halo of 2 rows and 4 columns (whole 4-byte words: the
+    // width is a multiple of 8, so a word lies entirely inside or outside a row)
+    constexpr int TW = 64 + 8, TH = BINS_TILE_H + 4, TWW = TW / 4, NW = TWW * TH;
+    constexpr int ROWS = BINS_TILE_H / 4;  // pixels per thread, one per row
     __shared__ float lut[256];
     __shared__ unsigned h[256];
     __shared__ float gt[TW * TH];
+    __shared__ float cal[96];
+    const int W = d.W, H = d.H;
+    const int lane = lane_id(), wave = threadIdx.x >> 6;
+    const int x = blockIdx.x * 64 + lane;
+    const int y0 = blockIdx.y * BINS_TILE_H + wave * ROWS;
+    const RT_GLOBAL uint16_t* d16 = gmem(d.d16[0]);
+    const RT_GLOBAL uint8_t* g8 = gmem(d.g8[0]);
+    const int tx0 = (int)blockIdx.x * 64 - 4, ty0 = (int)blockIdx.y * BINS_TILE_H - 2;
+    // Every global load of the block is issued before the first wait: the
+    // loads are unconditional, at coordinates clamped into the image, and
+    // what lies outside is discarded afterwards.
+    const int xc = min(x, W - 1);
+    unsigned zr[ROWS];
+#pragma unroll
+    for (int k = 0; k < ROWS; k++) zr[k] = d16[(size_t)xc + (size_t)min(y0 + k, H - 1) * W];
+    const bool aligned = ((uintptr_t)d.g8[0] & 3) == 0;
+    constexpr int NR = (NW + 255) / 256;
+    unsigned wv[NR];
+    bool win[NR];
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int e = min((int)threadIdx.x + r * 256, NW - 1);
+        const int gx = tx0 + 4 * (e % TWW), gy = ty0 + e / TWW;
+        win[r] = gx >= 0 && gx < W && gy >= 0 && gy < H;
+        const size_t o = (size_t)min(max(gx, 0), W - 4) + (size_t)min(max(gy, 0), H - 1) * W;
+        if (aligned) wv[r] = *(const RT_GLOBAL unsigned*)(g8 + o);
+        else wv[r] = g8[o] | (unsigned)g8[o + 1] << 8 | (unsigned)g8[o + 2] << 16 | (unsigned)g8[o + 3] << 24;
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) wv[r] = win[r] ? wv[r] : 0u;
     load_lut(lut);
     h[threadIdx.x] = 0u;
+    if (threadIdx.x < 96) cal[threadIdx.x] = d.calib[threadIdx.x];
     __syncthreads();
-    const int W = d.W, H = d.H;
-    const int tx0 = (int)blockIdx.x * 64 - 2, ty0 = (int)blockIdx.y * BINS_TILE_H - 2;
-    const RT_GLOBAL uint8_t* g8 = gmem(d.g8[0]);
-    for (int e = threadIdx.x; e < TW * TH; e += 256) {
-        int gx = tx0 + e % TW, gy = ty0 + e / TW;
-        gt[e] = (gx >= 0 && gx < W && gy >= 0 && gy < H) ? lut[g8[(size_t)gx + (size_t)gy * W]] : 0.0f;
+    // words outside the image are 0, and lut[0] = 0: those tile entries are
+    // never read (the bounds test keeps every tap 9 pixels inside the image)
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+        const int e = threadIdx.x + r * 256;
+        if (e < NW) {
+            float* q = gt + (e / TWW) * TW + 4 * (e % TWW);
+#pragma unroll
+            for (int j = 0; j < 4; j++) q[j] = lut[(wv[r] >> (8 * j)) & 0xffu];
+        }
     }
     __syncthreads();
     const LdsTile tile{gt, tx0, ty0, TW, TH};
-    const int lane = lane_id(), wave = threadIdx.x >> 6;
-    const int x = blockIdx.x * 64 + lane;
+    float cl[96];
+    uniform_calib(cal, cl);
     const U16DepthSrc zs{d.d16[0]};
     const U8Src gs{d.g8[0], lut};
     RT_GLOBAL uint8_t* bins = gmem(d.bins);
 #pragma unroll
-    for (int k = 0; k < BINS_TILE_H / 4; k++) {
-        const int y = blockIdx.y * BINS_TILE_H + wave * (BINS_TILE_H / 4) + k;
+    for (int k = 0; k < ROWS; k++) {
+        const int y = y0 + k;
         if (x < W && y < H) {
-            PixFeat f = pixel_features_tiled(zs, gs, tile, W, H, x, y, d.calib);
+            PixFeat f = pixel_features_tiled(-(float)zr[k], zs, gs, tile, W, H, x, y, cl);
             unsigned b = grad_bin(f.gmag);
             bins[(size_t)x + (size_t)y * W] = (uint8_t)b;
             atomicAdd(&h[b], 1u);
@@ -111,34 +148,67 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     if (v) atomicAdd(&d.ghist[threadIdx.x], v);
 }
 
-// seekThreshold256Kernel2 (filterIndex256.cu:26-46): threshold bin and excess,
-// once per pair.
-__global__ __launch_bounds__(256) void kb_select_thresh(const PairDesc* __restrict__ P) {
-    const PairDesc& d = P[blockIdx.x];
-    int thr, excess;
-    find_threshold(d.ghist, d.N, thr, excess);
-    if (threadIdx.x == 0) { d.seg[2560] = thr; d.seg[2561] = excess; }
+// Eight bins of one lane (pixels base .. base+7 of the segment [b0, end)) as
+// two 32-bit words. The loads are unconditional, from addresses clamped into
+// the segment, so that all of a wave's loads can be in flight together;
+// bytes beyond the segment are masked off by valid_bytes. vec: the segment
+// length is a multiple of 8, so one aligned 8-byte load covers them.
+struct Bins8 { unsigned lo, hi; };
+template <bool VEC>
+__device__ __forceinline__ Bins8 load_bins8(const RT_GLOBAL uint8_t* bins, int base, int end) {
+    Bins8 r;
+    if constexpr (VEC) {
+        typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+        u32x2 w = *(const RT_GLOBAL u32x2*)(bins + min(base, end - 8));
+        r.lo = w.x; r.hi = w.y;
+    } else {
+        unsigned v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) v[j] = bins[min(base + j, end - 1)];
+        r.lo = v[0] | v[1] << 8 | v[2] << 16 | v[3] << 24;
+        r.hi = v[4] | v[5] << 8 | v[6] << 16 | v[7] << 24;
+    }
+    return r;
 }
-
-// Four bins of one lane (pixels base .. base+3 of a segment), as above/equal flags.
-__device__ __forceinline__ void lane_bins4(const RT_GLOBAL uint8_t* bins, int base, int end, bool vec, unsigned thr,
-                                           unsigned& a4, unsigned& e4) {
-    a4 = 0u; e4 = 0u;
-    unsigned w = 0u;
+// the lane's 8 bins at offset o of segments s0, s0 + SEL_WAVES, ...; the
+// uniform branch sits outside, so each path issues all its loads back to back
+constexpr int SEL_BS = 1024, SEL_WAVES = SEL_BS / 64, SEL_U = 4;
+__device__ __forceinline__ void load_segments(Bins8* w, const RT_GLOBAL uint8_t* bins, int s0, int Pn, int o, bool vec) {
     if (vec) {
-        if (base < end) w = *(const RT_GLOBAL unsigned*)(bins + base);
+#pragma unroll
+        for (int u = 0; u < SEL_U; u++) {
+            const int b0 = (s0 + u * SEL_WAVES) * Pn;
+            w[u] = load_bins8<true>(bins, b0 + o, b0 + Pn);
+        }
     } else {
 #pragma unroll
-        for (int j = 0; j < 4; j++)
-            if (base + j < end) w |= (unsigned)bins[base + j] << (8 * j);
+        for (int u = 0; u < SEL_U; u++) {
+            const int b0 = (s0 + u * SEL_WAVES) * Pn;
+            w[u] = load_bins8<false>(bins, b0 + o, b0 + Pn);
+        }
     }
-#pragma unroll
-    for (int j = 0; j < 4; j++) {
-        if (base + j >= end) break;
-        unsigned bn = (w >> (8 * j)) & 0xffu;
-        a4 |= (unsigned)(bn > thr) << j;
-        e4 |= (unsigned)(bn == thr) << j;
-    }
+}
+// bit 7 of each of the first nv bytes of the lane's 8 (nv clamped to 0..8)
+__device__ __forceinline__ void valid_bytes(int nv, unsigned& vlo, unsigned& vhi) {
+    nv = max(0, min(nv, 8));
+    const int nl = min(nv, 4), nh = nv - nl;
+    vlo = nl == 4 ? 0x80808080u : 0x80808080u & ((1u << (8 * nl)) - 1u);
+    vhi = nh == 4 ? 0x80808080u : 0x80808080u & ((1u << (8 * nh)) - 1u);
+}
+// Byte-wise comparisons of a word with a threshold (SWAR; no carry or borrow
+// crosses a byte), the result in bit 7 of each byte.
+// bytes equal to t: T = t replicated
+__device__ __forceinline__ unsigned bytes_eq(unsigned w, unsigned T) {
+    const unsigned x = w ^ T;
+    const unsigned y = (x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu;  // bit 7 set where the low 7 bits of x are not all 0
+    return ~(y | x | 0x7F7F7F7Fu);
+}
+// bytes >= u: U = u replicated (0 < u < 256). The low 7 bits compare through
+// (w | 0x80) - (u & 0x7f), whose bit 7 is (w & 0x7f) >= (u & 0x7f); the top
+// bits decide when they differ.
+__device__ __forceinline__ unsigned bytes_ge(unsigned w, unsigned U) {
+    const unsigned d = (w | 0x80808080u) - (U & 0x7F7F7F7Fu);
+    return ((w & ~U) | (~(w ^ U) & d)) & 0x80808080u;
 }
 // inclusive sum over the lanes of a wave
 __device__ __forceinline__ int wave_scan_incl(int v) {
@@ -150,80 +220,177 @@ __device__ __forceinline__ int wave_scan_incl(int v) {
     }
     return v;
 }
-
-// filterIndexKernel4 (filterIndex256.cu:110-169) pass 1 with one wave per
-// segment of npix/512 pixels: counts of bins above and equal to the threshold.
-// Each lane reads four bins at a time.
-__global__ __launch_bounds__(256) void kb_select_count(const PairDesc* __restrict__ P) {
-    const PairDesc& d = P[blockIdx.y];
-    const unsigned thr = (unsigned)d.seg[2560];
-    int wave = threadIdx.x >> 6, lane = lane_id();
-    int s = blockIdx.x * 4 + wave;
-    int npix = d.W * d.H, Pn = npix / 512, b0 = s * Pn, end = b0 + Pn;
-    const bool vec = (Pn & 3) == 0;
-    const RT_GLOBAL uint8_t* bins = gmem(d.bins);
-    int cA = 0, cE = 0;
-    for (int off = 0; off < Pn; off += 256) {
-        unsigned a4, e4;
-        lane_bins4(bins, b0 + off + 4 * lane, end, vec, thr, a4, e4);
-        cA += __popc(a4);
-        cE += __popc(e4);
-    }
-    cA = (int)wave_sum((float)cA);
-    cE = (int)wave_sum((float)cE);
-    if (lane == 0) {
-        d.seg[s] = cA;
-        d.seg[512 + s] = cE;
-    }
-}
-__global__ __launch_bounds__(512) void kb_select_scan(const PairDesc* __restrict__ P) {
-    select_scan_block(P[blockIdx.x].seg);
-}
-// pass 3: ordered scatter, one wave per segment, four bins per lane
-__global__ __launch_bounds__(256) void kb_select_scatter(const PairDesc* __restrict__ P) {
-    const PairDesc& d = P[blockIdx.y];
-    int wave = threadIdx.x >> 6, lane = lane_id();
-    int s = blockIdx.x * 4 + wave;
-    const int* seg = d.seg;
-    unsigned thr = (unsigned)seg[2560];
-    int startA = seg[1024 + s], startE = seg[1536 + s], mE = seg[2048 + s];
-    int npix = d.W * d.H, Pn = npix / 512, b0 = s * Pn, end = b0 + Pn, N = d.N;
-    const bool vec = (Pn & 3) == 0;
-    const RT_GLOBAL uint8_t* bins = gmem(d.bins);
-    RT_GLOBAL int* sel = gmem(d.sel);
-    int rA = 0, rE = 0;
-    for (int off = 0; off < Pn; off += 256) {
-        const int base = b0 + off + 4 * lane;
-        unsigned a4, e4;
-        lane_bins4(bins, base, end, vec, thr, a4, e4);
-        int na = __popc(a4), ne = __popc(e4);
-        int ia = wave_scan_incl(na), ie = wave_scan_incl(ne);
-        int pa = rA + ia - na, pe = rE + ie - ne;  // ranks of this lane's first hits
+// inclusive maximum over the lanes of a wave
+__device__ __forceinline__ int wave_scan_max(int v) {
+    const int lane = lane_id();
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-            if ((a4 >> j) & 1u) { int pos = startA + pa; if (pos < N) sel[pos] = base + j; pa++; }
-            if ((e4 >> j) & 1u) {
-                if (pe < mE) { int pos = startE + pe; if (pos < N) sel[pos] = base + j; }
-                pe++;
+    for (int o = 1; o < 64; o <<= 1) {
+        int u = __shfl_up(v, o, 64);
+        if (lane >= o) v = max(v, u);
+    }
+    return v;
+}
+__device__ __forceinline__ int wave_isum(int v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Point selection of one pair in one workgroup (seekThreshold256Kernel2 +
+// filterIndexKernel4, filterIndex256.cu:26-46,110-169): threshold bin and
+// excess from the histogram; per 512-pixel-group segment the counts of bins
+// above / equal to the threshold; the trim of the equal counts (Q1) and the
+// segment start offsets; then the ordered scatter. The segment tables stay in
+// LDS. Each wave takes every SEL_WAVES-th segment, SEL_U of them at a time so
+// that their loads are in flight together. Per-lane counts are packed as
+// above + (equal << 16): a segment holds at most 1024 pixels (checked at
+// preparation), so neither half overflows.
+// above- and equal-threshold flags (bit 7 of each byte) of the lane's 8 bins
+struct SelFlags { unsigned alo, ahi, elo, ehi; };
+__device__ __forceinline__ SelFlags sel_flags(Bins8 w, int nv, unsigned T, unsigned U, bool any_above) {
+    unsigned vlo, vhi;
+    valid_bytes(nv, vlo, vhi);
+    SelFlags f;
+    f.alo = any_above ? bytes_ge(w.lo, U) & vlo : 0u;
+    f.ahi = any_above ? bytes_ge(w.hi, U) & vhi : 0u;
+    f.elo = bytes_eq(w.lo, T) & vlo;
+    f.ehi = bytes_eq(w.hi, T) & vhi;
+    return f;
+}
+__global__ __launch_bounds__(SEL_BS) void kb_select(const PairDesc* __restrict__ P) {
+    const PairDesc& d = P[blockIdx.x];
+    __shared__ int cA[512], cE[512], sA[512], sE[512], pm[512];
+    __shared__ int wtot[3][8];
+    __shared__ int s_first;
+    int thr_i, excess;
+    find_threshold(d.ghist, d.N, thr_i, excess);
+    const unsigned T = (unsigned)thr_i * 0x01010101u;
+    const bool any_above = thr_i < 255;                      // no bin lies above 255
+    const unsigned U = (unsigned)(thr_i + 1) * 0x01010101u;  // "above" = ">= thr + 1"
+    const int t = threadIdx.x, wave = t >> 6, lane = lane_id();
+    const int npix = d.W * d.H, Pn = npix / 512, N = d.N;
+    const bool vec = (Pn & 7) == 0;
+    const RT_GLOBAL uint8_t* bins = gmem(d.bins);
+    if (t == 0) s_first = 512;
+    // pass 1: counts
+    for (int s0 = wave; s0 < 512; s0 += SEL_WAVES * SEL_U) {
+        int ae[SEL_U];
+#pragma unroll
+        for (int u = 0; u < SEL_U; u++) ae[u] = 0;
+        for (int off = 0; off < Pn; off += 512) {
+            Bins8 w[SEL_U];
+            load_segments(w, bins, s0, Pn, off + 8 * lane, vec);
+#pragma unroll
+            for (int u = 0; u < SEL_U; u++) {
+                SelFlags f = sel_flags(w[u], Pn - off - 8 * lane, T, U, any_above);
+                ae[u] += __popc(f.alo) + __popc(f.ahi) + ((__popc(f.elo) + __popc(f.ehi)) << 16);
             }
         }
-        rA += __shfl(ia, 63, 64);
-        rE += __shfl(ie, 63, 64);
+#pragma unroll
+        for (int u = 0; u < SEL_U; u++) {
+            int v = wave_isum(ae[u]);
+            if (lane == 0) { cA[s0 + u * SEL_WAVES] = v & 0xffff; cE[s0 + u * SEL_WAVES] = v >> 16; }
+        }
     }
-    if (s == 0) {
-        int total = seg[2563];
-        for (int k = (total > 0 ? total : 0) + lane; k < N; k += 64) sel[k] = 0;
+    __syncthreads();
+    // Trim of the equal counts (filterIndex256.cu:141-152). Sequentially, over
+    // segments j = 0..510 with a positive equal count m_j and a running excess
+    // T (initially the excess): if m_j >= T the segment loses T and the trim
+    // ends; otherwise the segment is emptied and T becomes m_j (the
+    // reference's update T += m_j - T). In parallel: T before j is the count
+    // of the previous non-empty segment (or the excess), the trim ends at the
+    // first non-empty j with m_j >= T, every non-empty segment before it is
+    // emptied; the last segment is never trimmed.
+    int m = 0, prevval = 0;
+    bool nz = false;
+    if (t < 512) {
+        m = cE[t];
+        nz = t < 511 && m > 0;
+        int v = wave_scan_max(nz ? t : -1);
+        if (lane == 63) wtot[0][wave] = v;
+        pm[t] = v;
     }
+    __syncthreads();
+    if (t < 512) {
+        int prev = -1;
+        for (int w = 0; w < wave; w++) prev = max(prev, wtot[0][w]);
+        if (t > 0) prev = max(prev, (t - 1) >> 6 == wave ? pm[t - 1] : -1);
+        prevval = prev >= 0 ? cE[prev] : excess;
+        if (excess > 0 && nz && m >= prevval) atomicMin(&s_first, t);
+    }
+    __syncthreads();
+    if (t < 512 && excess > 0 && nz && t <= s_first) m = t < s_first ? 0 : m - prevval;
+    // exclusive offsets: above-threshold indices first, then the kept equal ones
+    int ia = 0, ie = 0;
+    if (t < 512) {
+        ia = wave_scan_incl(cA[t]);
+        ie = wave_scan_incl(m);
+        if (lane == 63) { wtot[1][wave] = ia; wtot[2][wave] = ie; }
+    }
+    __syncthreads();
+    if (t < 512) {
+        int oa = 0, oe = 0, ta = 0;
+        for (int w = 0; w < 8; w++) {
+            if (w < wave) { oa += wtot[1][w]; oe += wtot[2][w]; }
+            ta += wtot[1][w];
+        }
+        sA[t] = oa + ia - cA[t];
+        sE[t] = ta + oe + ie - m;
+        cE[t] = m;
+    }
+    int total = 0;
+    for (int w = 0; w < 8; w++) total += wtot[1][w] + wtot[2][w];
+    __syncthreads();
+    // pass 2: ordered scatter; a lane's hits are ranked by a wave scan of the
+    // packed counts, then written in pixel order (bit 8j+7 is pixel j)
+    RT_GLOBAL int* sel = gmem(d.sel);
+    for (int s0 = wave; s0 < 512; s0 += SEL_WAVES * SEL_U) {
+        int run[SEL_U];
+#pragma unroll
+        for (int u = 0; u < SEL_U; u++) run[u] = 0;
+        for (int off = 0; off < Pn; off += 512) {
+            Bins8 w[SEL_U];
+            load_segments(w, bins, s0, Pn, off + 8 * lane, vec);
+#pragma unroll
+            for (int u = 0; u < SEL_U; u++) {
+                const int s = s0 + u * SEL_WAVES;
+                const int base = s * Pn + off + 8 * lane;
+                const SelFlags f = sel_flags(w[u], Pn - off - 8 * lane, T, U, any_above);
+                const uint64_t am = f.alo | (uint64_t)f.ahi << 32, em = f.elo | (uint64_t)f.ehi << 32;
+                const int v = __popcll(am) + (__popcll(em) << 16);
+                const int incl = wave_scan_incl(v);
+                const int ex = run[u] + incl - v;  // ranks of this lane's first hits, packed
+                int pa = ex & 0xffff, pe = ex >> 16;
+                const int startA = sA[s], startE = sE[s], mE = cE[s];
+                for (uint64_t b = am; b; b &= b - 1u) {
+                    int pos = startA + pa;
+                    if (pos < N) sel[pos] = base + (__builtin_ctzll(b) >> 3);
+                    pa++;
+                }
+                for (uint64_t b = em; b; b &= b - 1u) {
+                    if (pe < mE) { int pos = startE + pe; if (pos < N) sel[pos] = base + (__builtin_ctzll(b) >> 3); }
+                    pe++;
+                }
+                run[u] += __shfl(incl, 63, 64);
+            }
+        }
+    }
+    // a short selection leaves zeros behind the selected indices
+    for (int k = (total > 0 ? total : 0) + t; k < N; k += SEL_BS) sel[k] = 0;
 }
 __global__ __launch_bounds__(256) void kb_ref_points(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.y];
     __shared__ float lut[256];
+    __shared__ float cal[96];
     load_lut(lut);
+    if (threadIdx.x < 96) cal[threadIdx.x] = d.calib[threadIdx.x];
     __syncthreads();
     int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= d.N) return;
+    float cl[96];
+    uniform_calib(cal, cl);
     ref_point(i, U16DepthSrc{d.d16[0]}, U8Src{d.g8[0], lut}, d.gray[0][1], d.gray[0][2], d.gray[0][3], d.W, d.H, d.L,
-              d.N, d.calib, 1e-3f, d.sel, d.pts, d.jac[0], d.jac[1], d.jac[2], d.jac[3]);
+              d.N, cl, 1e-3f, d.sel, d.pts, d.jac[0], d.jac[1], d.jac[2], d.jac[3]);
 }
 __global__ void kb_reset_queue(int* q) {
     if (threadIdx.x == 0) q[0] = 0;
@@ -237,8 +404,6 @@ struct TrackCfg {
     float scale_out;
 };
 
-// wave-uniform copy of a value (all lanes hold the same): lives in a scalar register
-__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
 
 // RZ: residual and zWeight of each point kept in registers (0) or in LDS (1,
 // at most RZ_MAX points per pair), trading LDS for occupancy.
@@ -474,7 +639,7 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
     size_t per = 0;  // inputs are borrowed device pointers; the arena holds derived data only
     per += 2 * align256(npx * 4 / 3 + 64);      // two gray pyramids, levels 1..3
     per += align256(npx);                       // gradient-magnitude bins
-    per += align256(256 * 4) + align256(2600 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(jac_bytes((int)N));
+    per += align256(256 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(jac_bytes((int)N));
     b->pair_bytes = per;
     size_t total = per * (size_t)max_pairs;
     void* q = nullptr;
@@ -521,10 +686,7 @@ static void launch_prep(rt_batch* b, hipStream_t s) {
     kb_pyramid<<<g64, 256, 0, s>>>(b->desc_dev, 1);
     dim3 gb((b->maxW + 63) / 64, (b->maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
     kb_bins<<<gb, 256, 0, s>>>(b->desc_dev);
-    kb_select_thresh<<<n, 256, 0, s>>>(b->desc_dev);
-    kb_select_count<<<dim3(128, n), 256, 0, s>>>(b->desc_dev);
-    kb_select_scan<<<n, 512, 0, s>>>(b->desc_dev);
-    kb_select_scatter<<<dim3(128, n), 256, 0, s>>>(b->desc_dev);
+    kb_select<<<n, SEL_BS, 0, s>>>(b->desc_dev);
     kb_ref_points<<<dim3((b->maxN + 255) / 256, n), 256, 0, s>>>(b->desc_dev);
 }
 
@@ -698,7 +860,6 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
         d.z[0] = d.z[1] = nullptr;   // depth is read from the 16-bit input
         d.bins = (uint8_t*)(base + off); off += align256((size_t)b->max_w * b->max_h);
         d.ghist = (unsigned*)(base + off); off += align256(256 * 4);
-        d.seg = (int*)(base + off); off += align256(2600 * 4);
         size_t NN = (size_t)b->prm.n_select;
         d.sel = (int*)(base + off); off += align256(NN * 4);
         d.pts = (float4*)(base + off); off += align256(NN * 16);

@@ -80,6 +80,13 @@ __device__ __forceinline__ float2 ldg(const float2* p, size_t i) {
     return make_float2(v.x, v.y);
 }
 
+// stores of element i through the global address space
+__device__ __forceinline__ void stg(float4* p, size_t i, float4 v) {
+    ((RT_GLOBAL nfloat4*)p)[i] = nfloat4{v.x, v.y, v.z, v.w};
+}
+__device__ __forceinline__ void stg(float2* p, size_t i, float2 v) { ((RT_GLOBAL nfloat2*)p)[i] = nfloat2{v.x, v.y}; }
+__device__ __forceinline__ void stg(float* p, size_t i, float v) { gmem(p)[i] = v; }
+
 // Image sources: FP32 planes, or the raw inputs read through their exact
 // conversions: 8-bit gray as (float)g/255 from a 256-entry table, 16-bit depth
 // in mm as -(float)d (the values the conversion kernel would have stored).
@@ -111,14 +118,6 @@ __device__ __forceinline__ float bilerp(const S& img, int width, int xdi, int yd
 }
 __device__ __forceinline__ float bilerp(const float* __restrict__ img, int width, int xdi, int ydi, float fx, float fy) {
     return bilerp(F32Src{img}, width, xdi, ydi, fx, fy);
-}
-// The same with every tap clamped into the image: used where the reference's
-// taps can leave the image (the coarsest attribute level near the border).
-__device__ __forceinline__ float bilerp_clamped(const float* __restrict__ img, int w, int h, int xdi, int ydi,
-                                                float fx, float fy) {
-    int x0 = min(max(xdi, 0), w - 1), x1 = min(max(xdi + 1, 0), w - 1);
-    int y0 = min(max(ydi, 0), h - 1), y1 = min(max(ydi + 1, 0), h - 1);
-    return blend(img[x0 + y0 * w], img[x1 + y0 * w], img[x0 + y1 * w], img[x1 + y1 * w], fx, fy);
 }
 // convertZMapToXYZKernel (rgb2gray.cu:570-593): back-projection with KL,
 // valid iff -max < z < -min. Returns validity; p = 0 when invalid.
@@ -196,13 +195,14 @@ struct LdsTile {
     }
 };
 // pixel_features with the 4x4 tap neighbourhood read from an LDS tile when it
-// lies inside it (the same values, so the same results), else from the image.
+// lies inside it (the same values, so the same results), else from the image;
+// zc is the pixel's own depth value z[xi + yi*W], loaded by the caller.
 template <class ZS, class GS>
-__device__ __forceinline__ PixFeat pixel_features_tiled(const ZS& z, const GS& g0, const LdsTile& tile, int W, int H,
-                                                        int xi, int yi, const float* c) {
+__device__ __forceinline__ PixFeat pixel_features_tiled(float zc, const ZS& z, const GS& g0, const LdsTile& tile,
+                                                        int W, int H, int xi, int yi, const float* c) {
     PixFeat f;
     f3 p3;
-    backproject(z[(size_t)xi + (size_t)yi * W], xi, yi, c, p3);
+    backproject(zc, xi, yi, c, p3);
     if (p3.z < 0.0f) {
         f3 r3 = mul4(c + TLR, p3);
         f2 pu{r3.x / r3.z, r3.y / r3.z};
@@ -242,6 +242,16 @@ __device__ __forceinline__ PixFeat pixel_features_tiled(const ZS& z, const GS& g
 
 // gradient-magnitude bin (gpu_hist.cu:135, filterIndex256.cu:126), fused multiply-add
 __device__ __forceinline__ unsigned grad_bin(float g) { return (unsigned)__builtin_fmaf(g, 255.0f, 0.5f); }
+
+// wave-uniform copy of a value (all lanes hold the same): lives in a scalar register
+__device__ __forceinline__ float uni(float v) { return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v))); }
+// The 96-float calibration block, staged in LDS by the workgroup, as
+// wave-uniform values: the helpers index it with constants, so it stays in
+// scalar registers instead of being loaded per pixel.
+__device__ __forceinline__ void uniform_calib(const float* cal_lds, float* cl) {
+#pragma unroll
+    for (int k = 0; k < 96; k++) cl[k] = uni(cal_lds[k]);
+}
 
 // wave64 helpers
 __device__ __forceinline__ float wave_sum(float v) {

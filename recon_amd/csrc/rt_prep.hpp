@@ -226,7 +226,7 @@ __device__ __forceinline__ void ref_point(int i, const ZS& z, const GS& g0,
                                                     const int* __restrict__ sel, float4* __restrict__ pts,
                                                     float4* __restrict__ j0, float4* __restrict__ j1,
                                                     float4* __restrict__ j2, float4* __restrict__ j3) {
-    int idx = sel[i];
+    int idx = ldg(sel, i);
     int xi = idx % W, yi = idx / W;
     PixFeat f = pixel_features(z, g0, W, H, xi, yi, c);
     float gx[4] = {f.gx, 0.0f, 0.0f, 0.0f}, gy[4] = {f.gy, 0.0f, 0.0f, 0.0f}, col[4] = {f.c1, 0.0f, 0.0f, 0.0f};
@@ -243,12 +243,26 @@ __device__ __forceinline__ void ref_point(int i, const ZS& z, const GS& g0,
                 int wl = W >> l, hl = H >> l;
                 int xdi = (int)q.x, ydi = (int)q.y;
                 float fX = q.x - (float)xdi, fY = q.y - (float)ydi;
-                const float* img = lv[l];
-                float cc = bilerp_clamped(img, wl, hl, xdi, ydi, fX, fY);
-                float cN = bilerp_clamped(img, wl, hl, xdi, ydi - 1, fX, fY);
-                float cW = bilerp_clamped(img, wl, hl, xdi - 1, ydi, fX, fY);
-                float cE = bilerp_clamped(img, wl, hl, xdi + 1, ydi, fX, fY);
-                float cS = bilerp_clamped(img, wl, hl, xdi, ydi + 1, fX, fY);
+                // Five bilinear samples (centre, N, W, E, S) with every tap clamped
+                // into the image, because the reference's taps can leave the image
+                // at the coarsest level near the border; together they read 12
+                // distinct taps of the clamped 4x4 neighbourhood: load each once.
+                const RT_GLOBAL float* im = gmem(lv[l]);
+                int xs[4], ys[4];
+#pragma unroll
+                for (int a = 0; a < 4; a++) {
+                    xs[a] = min(max(xdi - 1 + a, 0), wl - 1);
+                    ys[a] = min(max(ydi - 1 + a, 0), hl - 1) * wl;
+                }
+                float a01 = im[xs[1] + ys[0]], a02 = im[xs[2] + ys[0]];
+                float a10 = im[xs[0] + ys[1]], a11 = im[xs[1] + ys[1]], a12 = im[xs[2] + ys[1]], a13 = im[xs[3] + ys[1]];
+                float a20 = im[xs[0] + ys[2]], a21 = im[xs[1] + ys[2]], a22 = im[xs[2] + ys[2]], a23 = im[xs[3] + ys[2]];
+                float a31 = im[xs[1] + ys[3]], a32 = im[xs[2] + ys[3]];
+                float cc = blend(a11, a12, a21, a22, fX, fY);  // at (xdi, ydi)
+                float cN = blend(a01, a02, a11, a12, fX, fY);  // (xdi, ydi - 1)
+                float cW = blend(a10, a11, a20, a21, fX, fY);  // (xdi - 1, ydi)
+                float cE = blend(a12, a13, a22, a23, fX, fY);  // (xdi + 1, ydi)
+                float cS = blend(a21, a22, a31, a32, fX, fY);  // (xdi, ydi + 1)
                 gx[l] = (cE - cW) / 2.0f;
                 gy[l] = (cS - cN) / 2.0f;
                 col[l] = cc;
@@ -286,13 +300,13 @@ __device__ __forceinline__ void ref_point(int i, const ZS& z, const GS& g0,
             J[l][k] = (dp2.x / sc) * gx[l] + (dp2.y / sc) * gy[l];
         }
     }
-    pts[i] = make_float4(f.p.x, f.p.y, f.p.z, 0.0f);
+    stg(pts, i, make_float4(f.p.x, f.p.y, f.p.z, 0.0f));
     float4* jl[4] = {j0, j1, j2, j3};
     for (int l = 0; l < L; l++) {
         JacView v = jac_view(jl[l], N);
-        v.a[i] = make_float4(J[l][0], J[l][1], J[l][2], J[l][3]);
-        v.b[i] = make_float2(J[l][4], J[l][5]);
-        v.iref[i] = col[l];
+        stg(v.a, i, make_float4(J[l][0], J[l][1], J[l][2], J[l][3]));
+        stg(v.b, i, make_float2(J[l][4], J[l][5]));
+        stg(v.iref, i, col[l]);
     }
 }
 
