@@ -44,6 +44,37 @@ def calib_block_np(width, height, fx=None, fy=None, cx=None, cy=None, zmin=300.0
     return c
 
 
+def calib_block_rig(width, height, zmin=300.0, zmax=7000.0):
+    """A calibration block of a two-camera rig: depth intrinsics KL and colour
+    intrinsics KR differ, the colour camera sits 25 mm beside the depth camera
+    with a small rotation (TLR, TRL = its inverse) and has radial distortion
+    (k1, k2, k5). Every transform of the tracking path is then non-trivial;
+    the synthetic frames themselves are rendered in one camera, so this block
+    serves kernel-against-oracle comparisons, not convergence tests."""
+    s = width / 640.0
+    c = np.zeros(96, np.float32)
+
+    def K_and_inv(fx, fy, cx, cy):
+        K = np.array([-fx, 0, cx, 0, fy, cy, 0, 0, 1], np.float32)
+        iK = np.array([np.float32(1) / K[0], 0, -K[2] / K[0], 0, np.float32(1) / K[4], -K[5] / K[4], 0, 0, 1],
+                      np.float32)
+        return K, iK
+    KR, iKR = K_and_inv(525.0 * s, 523.0 * s, 319.5 * s + 1.7, 239.5 * s - 2.1)
+    KL, iKL = K_and_inv(580.0 * s, 578.0 * s, 322.1 * s, 236.4 * s)
+    c[0:9] = KR; c[9:18] = iKR; c[43:52] = KL; c[52:61] = iKL
+    c[18:26] = np.array([0.04, -0.08, 0.0, 0.0, 0.01, 0, 0, 0], np.float32)
+    ay, ax = np.radians(0.6), np.radians(0.3)
+    Ry = np.array([[np.cos(ay), 0, np.sin(ay)], [0, 1, 0], [-np.sin(ay), 0, np.cos(ay)]])
+    Rx = np.array([[1, 0, 0], [0, np.cos(ax), -np.sin(ax)], [0, np.sin(ax), np.cos(ax)]])
+    T = np.eye(4)
+    T[:3, :3] = Ry @ Rx
+    T[:3, 3] = [25.0, 0.4, -1.2]
+    c[26:42] = T.astype(np.float32).ravel()
+    c[64:80] = np.linalg.inv(T).astype(np.float32).ravel()
+    c[62] = zmin; c[63] = zmax
+    return c
+
+
 class Scene:
     def __init__(self, seed):
         self._lib = synth_lib()

@@ -12,25 +12,28 @@ pytestmark = pytest.mark.gpu
 SIZES = [(160, 120), (320, 240), (480, 360), (640, 480)]
 
 
-def _batch(n, theta=3.0, tmax=50.0, seed=20):
+def _batch(n, theta=3.0, tmax=50.0, seed=20, rig=False):
+    """n pairs of mixed sizes; rig: every other pair under the two-camera rig
+    calibration (distortion, TLR/TRL, KL != KR) instead of the pinhole one."""
     pairs, host, gts = [], [], []
     for i in range(n):
         W, H = SIZES[i % len(SIZES)]
         g0, d0, g1, d1, Tgt = synth.make_pair(seed + i, W, H, theta, tmax)
         bufs = [DeviceBuffer.from_array(a) for a in (g0, d0, g1, d1)]
         host.append((g0, d0, g1, d1, bufs))
+        cal = synth.calib_block_rig(W, H) if rig and i % 2 else synth.calib_block_np(W, H)
         pairs.append(dict(ref_gray=bufs[0].ptr, ref_depth=bufs[1].ptr, cur_gray=bufs[2].ptr, cur_depth=bufs[3].ptr,
-                          width=W, height=H, calib=synth.calib_block_np(W, H)))
+                          width=W, height=H, calib=cal))
         gts.append(Tgt)
     return pairs, host, gts
 
 
-@pytest.mark.parametrize("robust,bins,solver", [(0, 64, 0), (1, 256, 1)])
-def test_batch_matches_single_pair(robust, bins, solver):
-    print(f"case batch robust {robust} bins {bins} solver {solver}", flush=True)
+@pytest.mark.parametrize("robust,bins,solver,rig", [(0, 64, 0, False), (1, 256, 1, False), (0, 64, 0, True)])
+def test_batch_matches_single_pair(robust, bins, solver, rig):
+    print(f"case batch robust {robust} bins {bins} solver {solver} rig {rig}", flush=True)
     n = 8
     N = 2048
-    pairs, host, gts = _batch(n)
+    pairs, host, gts = _batch(n, rig=rig)
     bt = BatchTracker(n, 640, 480, mode=PHOTOMETRIC, n_select=N, robust=robust, mad_bins=bins, solver=solver)
     Trel, T, rms, st = bt.track(pairs)
     assert np.all(st == 0)
@@ -44,7 +47,8 @@ def test_batch_matches_single_pair(robust, bins, solver):
         tr.track()
         Ts = tr.trel().astype(np.float64)
         rot, tmm = synth.pose_error(Trel[i], Ts)
-        print(f"  pair {i} {W}x{H}: batch vs single {rot:.2e} deg {tmm:.2e} mm; vs truth {synth.pose_error(Trel[i], gts[i])}")
+        print(f"  pair {i} {W}x{H}: batch vs single {rot:.2e} deg {tmm:.2e} mm; vs truth {synth.pose_error(Trel[i], gts[i])}",
+              flush=True)
         assert np.radians(rot) < 1e-4 and tmm < 5e-2
         assert np.allclose(T[i] @ Trel[i], np.eye(4), atol=1e-5)
 
@@ -71,15 +75,16 @@ def test_batch_retrack_deterministic_and_warm_start():
     assert checked >= 2
 
 
-@pytest.mark.parametrize("N", [2048])
-def test_batch_preparation_bitexact_against_oracle(N):
+@pytest.mark.parametrize("N,rig", [(2048, False), (2048, True)])
+def test_batch_preparation_bitexact_against_oracle(N, rig):
     """The batch path's keyframe preparation (gradient bins from LDS-staged
     tiles, histogram, selection, points, Jacobians) equals the FP32 oracle bit
-    for bit, pair by pair, on a mixed-size batch."""
-    print(f"case batch preparation N={N}", flush=True)
+    for bit, pair by pair, on a mixed-size batch (rig: half of the pairs under
+    the two-camera rig calibration)."""
+    print(f"case batch preparation N={N} rig {rig}", flush=True)
     from tests._oracle_steps import Steps
     n = 4
-    pairs, host, gts = _batch(n, seed=60)
+    pairs, host, gts = _batch(n, seed=60, rig=rig)
     bt = BatchTracker(n, 640, 480, mode=PHOTOMETRIC, n_select=N)
     bt.prepare_only(bt.prepare(pairs))
     for i in range(n):

@@ -18,15 +18,18 @@ from tests._oracle_steps import Steps
 pytestmark = pytest.mark.gpu
 
 CASES = [(320, 240, 3, 8192, 1), (640, 480, 4, 8192, 2), (160, 120, 3, 2048, 3)]
+# the same stages under a two-camera rig calibration (distortion, TLR/TRL, KL != KR)
+RIG_CASES = [(320, 240, 3, 8192, 5), (640, 480, 4, 8192, 6)]
+CALIBS = {"pinhole": synth.calib_block_np, "rig": synth.calib_block_rig}
 
 
 def _pair(W, H, seed, theta=2.0, t=30.0):
     return synth.make_pair(seed, W, H, theta, t)
 
 
-def _setup(W, H, L, N, seed, mode=PHOTOMETRIC, **kw):
+def _setup(W, H, L, N, seed, mode=PHOTOMETRIC, calib="pinhole", **kw):
     g0, d0, g1, d1, Tgt = _pair(W, H, seed)
-    cal = synth.calib_block_np(W, H)
+    cal = CALIBS[calib](W, H)
     tr = PhotoTracker(W, H, cal, mode=mode, levels=L, n_select=N, **kw)
     tr.set_reference_points(g0, d0, mode=mode, lam=0.5, force=True)
     tr.set_current_image(g1, d1)
@@ -72,10 +75,10 @@ def test_rgbd_texture_input_bitexact():
     assert np.array_equal(tr.debug_read("CUR_Z", 0, np.float32, W * H), z)
 
 
-@pytest.mark.parametrize("W,H,L,N,seed", CASES)
-def test_selection_bitexact(W, H, L, N, seed):
-    print(f"case selection {W}x{H} seed {seed}", flush=True)
-    tr, st, _ = _setup(W, H, L, N, seed)
+@pytest.mark.parametrize("W,H,L,N,seed,calib", [c + ("pinhole",) for c in CASES] + [c + ("rig",) for c in RIG_CASES])
+def test_selection_bitexact(W, H, L, N, seed, calib):
+    print(f"case selection {W}x{H} seed {seed} {calib}", flush=True)
+    tr, st, _ = _setup(W, H, L, N, seed, calib=calib)
     tr.debug_prepare()
     st.features()
     assert np.array_equal(tr.debug_read("GRAD_MAG", 0, np.float32, W * H), st.gmag)
@@ -83,10 +86,10 @@ def test_selection_bitexact(W, H, L, N, seed):
     assert np.array_equal(tr.debug_read("SELECTED", 0, np.int32, N), st.sel)
 
 
-@pytest.mark.parametrize("W,H,L,N,seed", CASES)
-def test_points_and_jacobians_bitexact(W, H, L, N, seed):
-    print(f"case jacobians {W}x{H} seed {seed}", flush=True)
-    tr, st, _ = _setup(W, H, L, N, seed)
+@pytest.mark.parametrize("W,H,L,N,seed,calib", [c + ("pinhole",) for c in CASES] + [c + ("rig",) for c in RIG_CASES])
+def test_points_and_jacobians_bitexact(W, H, L, N, seed, calib):
+    print(f"case jacobians {W}x{H} seed {seed} {calib}", flush=True)
+    tr, st, _ = _setup(W, H, L, N, seed, calib=calib)
     tr.debug_prepare()
     st.features()
     pts = tr.debug_read("POINTS", 0, np.float32, 4 * N).reshape(N, 4)
@@ -101,11 +104,11 @@ def test_points_and_jacobians_bitexact(W, H, L, N, seed):
         assert np.array_equal(jac[6 * N:], rec[:, color_slot[l]]), f"reference gray level {l}"
 
 
-@pytest.mark.parametrize("W,H,L,N,seed", CASES)
+@pytest.mark.parametrize("W,H,L,N,seed,calib", [c + ("pinhole",) for c in CASES] + [c + ("rig",) for c in RIG_CASES])
 @pytest.mark.parametrize("bins,robust", [(64, 0), (256, 0), (64, 1)])
-def test_photometric_linearisation(W, H, L, N, seed, bins, robust):
-    print(f"case photo-lin {W}x{H} seed {seed} bins {bins} robust {robust}", flush=True)
-    tr, st, Tgt = _setup(W, H, L, N, seed, mad_bins=bins, robust=robust)
+def test_photometric_linearisation(W, H, L, N, seed, calib, bins, robust):
+    print(f"case photo-lin {W}x{H} seed {seed} {calib} bins {bins} robust {robust}", flush=True)
+    tr, st, Tgt = _setup(W, H, L, N, seed, calib=calib, mad_bins=bins, robust=robust)
     tr.debug_prepare()
     st.features()
     T = _trel_guess(Tgt)
@@ -122,10 +125,10 @@ def test_photometric_linearisation(W, H, L, N, seed, bins, robust):
         assert np.all(np.abs(bp - o["b"]) <= tol * o["babs"] + 1e-30), f"Jtr level {layer}"
 
 
-@pytest.mark.parametrize("W,H,L,N,seed", CASES[:2])
-def test_icp_clouds_bitexact(W, H, L, N, seed):
-    print(f"case clouds {W}x{H} seed {seed}", flush=True)
-    tr, st, _ = _setup(W, H, L, N, seed, mode=GEOMETRIC)
+@pytest.mark.parametrize("W,H,L,N,seed,calib", [c + ("pinhole",) for c in CASES[:2]] + [RIG_CASES[0] + ("rig",)])
+def test_icp_clouds_bitexact(W, H, L, N, seed, calib):
+    print(f"case clouds {W}x{H} seed {seed} {calib}", flush=True)
+    tr, st, _ = _setup(W, H, L, N, seed, mode=GEOMETRIC, calib=calib)
     tr.debug_prepare()
     st.clouds()
     for l in range(L):
@@ -144,10 +147,10 @@ def test_icp_clouds_bitexact(W, H, L, N, seed):
         assert np.array_equal(cc[mc, :3], oc[mc, :3]), f"cur points level {l}"
 
 
-@pytest.mark.parametrize("W,H,L,N,seed", CASES[:2])
-def test_icp_selection_weights_and_system(W, H, L, N, seed):
-    print(f"case icp {W}x{H} seed {seed}", flush=True)
-    tr, st, Tgt = _setup(W, H, L, N, seed, mode=GEOMETRIC)
+@pytest.mark.parametrize("W,H,L,N,seed,calib", [c + ("pinhole",) for c in CASES[:2]] + [RIG_CASES[0] + ("rig",)])
+def test_icp_selection_weights_and_system(W, H, L, N, seed, calib):
+    print(f"case icp {W}x{H} seed {seed} {calib}", flush=True)
+    tr, st, Tgt = _setup(W, H, L, N, seed, mode=GEOMETRIC, calib=calib)
     tr.debug_prepare()
     st.clouds()
     T = _trel_guess(Tgt)
