@@ -13,6 +13,7 @@
 // picked per batch by a timing autotuner; the pipeline is captured once into a
 // hipGraph and replayed.
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -133,16 +134,20 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     const U16DepthSrc zs{d.d16[0]};
     const U8Src gs{d.g8[0], lut};
     RT_GLOBAL uint8_t* bins = gmem(d.bins);
+    auto pixels = [&](auto simple) {
 #pragma unroll
-    for (int k = 0; k < ROWS; k++) {
-        const int y = y0 + k;
-        if (x < W && y < H) {
-            PixFeat f = pixel_features_tiled(-(float)zr[k], zs, gs, tile, W, H, x, y, cl);
-            unsigned b = grad_bin(f.gmag);
-            bins[(size_t)x + (size_t)y * W] = (uint8_t)b;
-            atomicAdd(&h[b], 1u);
+        for (int k = 0; k < ROWS; k++) {
+            const int y = y0 + k;
+            if (x < W && y < H) {
+                PixFeat f = pixel_features_tiled<decltype(simple)::value>(-(float)zr[k], zs, gs, tile, W, H, x, y, cl);
+                unsigned b = grad_bin(f.gmag);
+                bins[(size_t)x + (size_t)y * W] = (uint8_t)b;
+                atomicAdd(&h[b], 1u);
+            }
         }
-    }
+    };
+    if (calib_simple(cl)) pixels(std::true_type{});
+    else pixels(std::false_type{});
     __syncthreads();
     unsigned v = h[threadIdx.x];
     if (v) atomicAdd(&d.ghist[threadIdx.x], v);
@@ -445,6 +450,7 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
         float cl[96];
 #pragma unroll
         for (int k = 0; k < 96; k++) cl[k] = uni(cal[k]);
+        const bool simple = calib_simple(cl);
         float rms_last = 0.0f;
         for (int layer = L - 1; layer >= 0; layer--) {
             const int nit = d.iters[L - 1 - layer];
@@ -461,12 +467,18 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
                 // residual pass: r and zWeight stay in registers (RZ 0) or go to LDS (RZ 1)
                 float r[RZ ? 1 : PPT], zw[RZ ? 1 : PPT];
                 auto residual = [&](int i, float& rr, float& zz) {
-                    if (layer == 0)
-                        photo_point(ldg(d.pts, i), ldg(jv.iref, i), Tl, a, b, U8Src{d.g8[1], lut}, wl, hl,
-                                    U16DepthSrc{d.d16[1]}, W, H, cl, rr, zz);
-                    else
-                        photo_point(ldg(d.pts, i), ldg(jv.iref, i), Tl, a, b, F32Src{img}, wl, hl,
-                                    U16DepthSrc{d.d16[1]}, W, H, cl, rr, zz);
+                    const float4 p4 = ldg(d.pts, i);
+                    const float ir = ldg(jv.iref, i);
+                    const U16DepthSrc zs{d.d16[1]};
+                    if (layer == 0) {
+                        const U8Src g{d.g8[1], lut};
+                        if (simple) photo_point<true>(p4, ir, Tl, a, b, g, wl, hl, zs, W, H, cl, rr, zz);
+                        else photo_point<false>(p4, ir, Tl, a, b, g, wl, hl, zs, W, H, cl, rr, zz);
+                    } else {
+                        const F32Src g{img};
+                        if (simple) photo_point<true>(p4, ir, Tl, a, b, g, wl, hl, zs, W, H, cl, rr, zz);
+                        else photo_point<false>(p4, ir, Tl, a, b, g, wl, hl, zs, W, H, cl, rr, zz);
+                    }
                     int bn = (int)((float)(cfg.mad_bins - 1) * fabsf(rr));
                     if (bn >= 0 && bn < cfg.mad_bins) atomicAdd(&hist[bn], 1u);
                 };

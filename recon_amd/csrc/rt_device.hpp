@@ -57,6 +57,31 @@ __device__ __forceinline__ f2 distort(f2 pu, const float* kc, const float* K) {
 __device__ __forceinline__ bool in_bounds(f2 p, int w, int h) {
     return p.x >= 10.0f && p.x < (float)(w - 10) && p.y >= 10.0f && p.y < (float)(h - 10);
 }
+// A calibration whose colour camera coincides with the depth camera: TLR and
+// TRL are identities, no radial distortion (k1 = k2 = k5 = 0), KR = KL and a
+// pinhole without skew. Every transform skipped under it is an exact identity
+// (x*1 + y*0 + z*0 + 0 rounds to x, 1 + 0*r2 + 0*r4 + 0*r6 to 1, and the
+// depth-camera reprojection of a colour-camera point is that point), so the
+// kernels may take the shorter path and compute the same values; the only
+// possible difference, the sign of a zero coordinate, is mapped to the same
+// pixel position by every later use (cx, cy are nonzero).
+__device__ __forceinline__ bool calib_simple(const float* c) {
+    bool s = c[KCR] == 0.0f && c[KCR + 1] == 0.0f && c[KCR + 4] == 0.0f;
+#pragma unroll
+    for (int k = 0; k < 12; k++) {
+        const float id = (k % 5 == 0) ? 1.0f : 0.0f;
+        s = s && c[TLR + k] == id && c[TRL + k] == id;
+    }
+#pragma unroll
+    for (int k = 0; k < 9; k++) s = s && c[KR + k] == c[KL + k];
+    return s && c[KR + 1] == 0.0f && c[KR + 3] == 0.0f && c[KR + 6] == 0.0f && c[KR + 7] == 0.0f &&
+           c[KR + 8] == 1.0f && c[KR + 2] != 0.0f && c[KR + 5] != 0.0f;
+}
+// distortPoint with zero distortion: K applied to the normalised point
+__device__ __forceinline__ f2 project_simple(f2 pu, const float* K) {
+    return f2{K[0] * pu.x + K[2], K[4] * pu.y + K[5]};
+}
+
 // Pointers into device memory (global address space): loads through them are
 // global_load with a scalar base, not flat loads, which would also wait on the
 // LDS counter and need 64-bit addresses per lane.
@@ -197,16 +222,17 @@ struct LdsTile {
 // pixel_features with the 4x4 tap neighbourhood read from an LDS tile when it
 // lies inside it (the same values, so the same results), else from the image;
 // zc is the pixel's own depth value z[xi + yi*W], loaded by the caller.
-template <class ZS, class GS>
+// SIMPLE: the calibration satisfies calib_simple.
+template <bool SIMPLE, class ZS, class GS>
 __device__ __forceinline__ PixFeat pixel_features_tiled(float zc, const ZS& z, const GS& g0, const LdsTile& tile,
                                                         int W, int H, int xi, int yi, const float* c) {
     PixFeat f;
     f3 p3;
     backproject(zc, xi, yi, c, p3);
     if (p3.z < 0.0f) {
-        f3 r3 = mul4(c + TLR, p3);
+        f3 r3 = SIMPLE ? p3 : mul4(c + TLR, p3);
         f2 pu{r3.x / r3.z, r3.y / r3.z};
-        f2 p = distort(pu, c + KCR, c + KR);
+        f2 p = SIMPLE ? project_simple(pu, c + KR) : distort(pu, c + KCR, c + KR);
         if (in_bounds(p, W, H)) {
             int xdi = (int)p.x, ydi = (int)p.y;
             float fx = p.x - (float)xdi, fy = p.y - (float)ydi;

@@ -487,15 +487,18 @@ __device__ bool solve_update(const double* sys, float* trel, const SolveCfg& cfg
 // bilerp(I_cur(l)) (1.0 outside the view) and the zWeight from the nearest
 // full-resolution current depth. a = 2^-l, b = (a-1)/2 (interpolateResidual2,
 // warp.cu:1209-1211).
-template <class GS, class ZS>
+// SIMPLE: the calibration satisfies calib_simple, so r3 = T p, no distortion,
+// and the depth-camera reprojection of the zWeight is the level-0 projection
+// p2 itself.
+template <bool SIMPLE = false, class GS, class ZS>
 __device__ __forceinline__ void photo_point(float4 p4, float iref, const float* T, float a, float b, const GS& img,
                                             int wl, int hl, const ZS& zcur, int W, int H, const float* c, float& ir,
                                             float& zW) {
     f3 p3{p4.x, p4.y, p4.z};
     f3 r3ir = mul4(T, p3);
-    f3 r3 = mul4(c + TLR, r3ir);
+    f3 r3 = SIMPLE ? r3ir : mul4(c + TLR, r3ir);
     f2 pu{r3.x / r3.z, r3.y / r3.z};
-    f2 p2 = distort(pu, c + KCR, c + KR);
+    f2 p2 = SIMPLE ? project_simple(pu, c + KR) : distort(pu, c + KCR, c + KR);
     f2 p{a * p2.x + b, a * p2.y + b};
     ir = 1.0f;
     zW = 0.0f;
@@ -507,9 +510,14 @@ __device__ __forceinline__ void photo_point(float4 p4, float iref, const float* 
         float fx = p.x - (float)xdi, fy = p.y - (float)ydi;
         float color = bilerp(img, wl, xdi, ydi, fx, fy);
         ir = iref - color;
-        f3 rl3 = mul4(c + TRL, r3);
-        rl3.x /= rl3.z; rl3.y /= rl3.z; rl3.z = 1.0f;
-        f3 pl2 = mul3(c + KL, rl3);
+        f3 pl2;
+        if (SIMPLE) {
+            pl2 = f3{p2.x, p2.y, 1.0f};
+        } else {
+            f3 rl3 = mul4(c + TRL, r3);
+            rl3.x /= rl3.z; rl3.y /= rl3.z; rl3.z = 1.0f;
+            pl2 = mul3(c + KL, rl3);
+        }
         bool fin2 = fabsf(pl2.x) < 1e9f && fabsf(pl2.y) < 1e9f;
         int xdi2 = fin2 ? (int)(pl2.x + 0.5f) : -1, ydi2 = fin2 ? (int)(pl2.y + 0.5f) : -1;
         if (xdi2 >= 0 && ydi2 >= 0 && xdi2 < W && ydi2 < H) {
