@@ -34,6 +34,7 @@ struct PairDesc {
     int W, H, L, N;
     int iters[4];
     const float* calib;
+    int simple;             // calib_simple(calib), evaluated at preparation
     const uint8_t* g8[2];
     const uint16_t* d16[2];
     float* gray[2][4];
@@ -53,8 +54,17 @@ struct PairDesc {
 // Level 0 of both frames stays in its input form (8-bit gray, 16-bit depth):
 // every consumer reads it through the exact conversions (U8Src, U16DepthSrc),
 // so no FP32 copy is written or read back.
+// (float)t / 255 for every 8-bit value, evaluated by the compiler (IEEE
+// division, round to nearest: the values the device division gives)
+struct U8Table { float v[256]; };
+constexpr U8Table make_u8_table() {
+    U8Table l{};
+    for (int t = 0; t < 256; t++) l.v[t] = (float)t / 255.0f;
+    return l;
+}
+__constant__ U8Table c_u8_table = make_u8_table();
 __device__ __forceinline__ void load_lut(float* lut) {
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) lut[t] = (float)t / 255.0f;
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) lut[t] = c_u8_table.v[t];
 }
 
 // pyramid levels 1..L-1 of one frame (fastDownSampleHdrKernel, downSample2Cuda.cu:35-42)
@@ -80,7 +90,10 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     constexpr int TW = 64 + 8, TH = BINS_TILE_H + 4, TWW = TW / 4, NW = TWW * TH;
     constexpr int ROWS = BINS_TILE_H / 4;  // pixels per thread, one per row
     __shared__ float lut[256];
-    __shared__ unsigned h[256];
+    // four histogram copies, by lane: lanes with equal bins mostly update
+    // different addresses (neighbouring pixels tend to share a bin)
+    constexpr int HC = 4;
+    __shared__ unsigned h[256 * HC];
     __shared__ float gt[TW * TH];
     __shared__ float cal[96];
     const int W = d.W, H = d.H;
@@ -113,7 +126,8 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
 #pragma unroll
     for (int r = 0; r < NR; r++) wv[r] = win[r] ? wv[r] : 0u;
     load_lut(lut);
-    h[threadIdx.x] = 0u;
+#pragma unroll
+    for (int k = 0; k < HC; k++) h[threadIdx.x + 256 * k] = 0u;
     if (threadIdx.x < 96) cal[threadIdx.x] = d.calib[threadIdx.x];
     __syncthreads();
     // words outside the image are 0, and lut[0] = 0: those tile entries are
@@ -142,14 +156,16 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
                 PixFeat f = pixel_features_tiled<decltype(simple)::value>(-(float)zr[k], zs, gs, tile, W, H, x, y, cl);
                 unsigned b = grad_bin(f.gmag);
                 bins[(size_t)x + (size_t)y * W] = (uint8_t)b;
-                atomicAdd(&h[b], 1u);
+                atomicAdd(&h[b * HC + (lane & (HC - 1))], 1u);
             }
         }
     };
-    if (calib_simple(cl)) pixels(std::true_type{});
+    if (d.simple) pixels(std::true_type{});
     else pixels(std::false_type{});
     __syncthreads();
-    unsigned v = h[threadIdx.x];
+    unsigned v = 0u;
+#pragma unroll
+    for (int k = 0; k < HC; k++) v += h[threadIdx.x * HC + k];
     if (v) atomicAdd(&d.ghist[threadIdx.x], v);
 }
 
@@ -450,7 +466,7 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
         float cl[96];
 #pragma unroll
         for (int k = 0; k < 96; k++) cl[k] = uni(cal[k]);
-        const bool simple = calib_simple(cl);
+        const bool simple = d.simple != 0;
         float rms_last = 0.0f;
         for (int layer = L - 1; layer >= 0; layer--) {
             const int nit = d.iters[L - 1 - layer];
@@ -857,6 +873,7 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
         for (int k = 0; k < 4; k++) d.iters[k] = q.iters[k];
         d.calib = b->calib_dev + 96 * (size_t)i;
         memcpy(&calib[96 * (size_t)i], p.calib, 96 * 4);
+        d.simple = calib_simple(p.calib) ? 1 : 0;
         d.g8[0] = p.ref_gray; d.d16[0] = p.ref_depth; d.g8[1] = p.cur_gray; d.d16[1] = p.cur_depth;
         char* base = b->arena + b->pair_bytes * (size_t)i;
         size_t npx = (size_t)p.width * p.height, off = 0;

@@ -65,7 +65,7 @@ __device__ __forceinline__ bool in_bounds(f2 p, int w, int h) {
 // kernels may take the shorter path and compute the same values; the only
 // possible difference, the sign of a zero coordinate, is mapped to the same
 // pixel position by every later use (cx, cy are nonzero).
-__device__ __forceinline__ bool calib_simple(const float* c) {
+__host__ __device__ __forceinline__ bool calib_simple(const float* c) {
     bool s = c[KCR] == 0.0f && c[KCR + 1] == 0.0f && c[KCR + 4] == 0.0f;
 #pragma unroll
     for (int k = 0; k < 12; k++) {
