@@ -572,10 +572,12 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
                         if (i < N) accumulate(i, r[k], zw[k]);
                     }
                 }
+                {
+                    float v32[32];
 #pragma unroll
-                for (int k = 0; k < 29; k++) {
-                    float v = wave_sum(acc[k]);
-                    if (lane == 0) wsum[wave][k] = (double)v;
+                    for (int k = 0; k < 32; k++) v32[k] = k < 29 ? acc[k] : 0.0f;
+                    const float tot = wave_sum32(v32);
+                    if ((lane & 1) == 0 && (lane >> 1) < 29) wsum[wave][lane >> 1] = (double)tot;
                 }
                 __syncthreads();
                 // wave sums combined in FP64 in wave order, one term per lane

@@ -286,5 +286,41 @@ __device__ __forceinline__ float wave_sum(float v) {
     return v;
 }
 __device__ __forceinline__ int lane_id() { return (int)__lane_id(); }
+// Sums of 32 per-lane values over the 64 lanes of a wave, by halving: at the
+// step with lane offset o every lane keeps half of its remaining values (the
+// upper half when lane & o) and adds its partner's copy of that half, so 32
+// values need 16+8+4+2+1+1 exchanges instead of 32 six-step trees. The first
+// two steps are v_permlane32_swap / v_permlane16_swap, which exchange lanes
+// 32-63 of one register with lanes 0-31 of another (rows 1, 3 with rows 0, 2
+// of 16 lanes): after the swap the two registers hold, lane by lane, the
+// kept value and the partner's copy of it. On return lanes 2k and 2k+1 hold
+// the total of value k.
+__device__ __forceinline__ float swap_add32(float x, float y) {
+    auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float swap_add16(float x, float y) {
+    auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(y), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+__device__ __forceinline__ float wave_sum32(const float* v) {
+    const int lane = lane_id();
+    float a[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) a[j] = swap_add32(v[j], v[16 + j]);   // lane & 32 ? value 16 + j : value j
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = swap_add16(a[j], a[8 + j]);     // lane & 16 ? 8 + j : j
+#pragma unroll
+    for (int n = 4; n >= 1; n >>= 1) {  // offsets 8, 4, 2: both sums, then a select of values
+        const bool up = lane & (2 * n);
+#pragma unroll
+        for (int j = 0; j < n; j++) {
+            const float x = a[j], y = a[n + j];
+            const float sx = x + __shfl_xor(x, 2 * n, 64), sy = y + __shfl_xor(y, 2 * n, 64);
+            a[j] = up ? sy : sx;
+        }
+    }
+    return a[0] + __shfl_xor(a[0], 1, 64);
+}
 
 }  // namespace rt
