@@ -429,9 +429,14 @@ struct TrackCfg {
 // RZ: residual and zWeight of each point kept in registers (0) or in LDS (1,
 // at most RZ_MAX points per pair), trading LDS for occupancy.
 constexpr int RZ_MAX = 8192;
-template <int BS, int PPT, int RZ>
-__global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, const int* __restrict__ order,
-                                               int npairs, int* queue, TrackCfg cfg) {
+// WPE: waves per SIMD the register allocation must allow (1: no constraint
+// beyond the workgroup size). Two 512-thread workgroups per CU (WPE 4), which
+// would overlap one pair's serial phases with the other's point passes, were
+// measured slower than one 1024-thread workgroup at 640x480: the kernel is
+// bound by memory traffic, not by those phases.
+template <int BS, int PPT, int RZ, int WPE = 1>
+__global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__ P, const int* __restrict__ order,
+                                                    int npairs, int* queue, TrackCfg cfg) {
     constexpr int NW = BS / 64;
     __shared__ float cal[96];
     __shared__ float T[16];
@@ -603,15 +608,18 @@ __global__ __launch_bounds__(BS) void kb_track(const PairDesc* __restrict__ P, c
 }
 
 typedef void (*TrackFn)(const PairDesc*, const int*, int, int*, TrackCfg);
-struct Variant { int bs, ppt, rz; TrackFn fn; };
+struct Variant { int bs, ppt, rz, wpe; TrackFn fn; };
 static const Variant kVariants[] = {
-    {256, 8, 0, kb_track<256, 8, 0>},     {256, 16, 0, kb_track<256, 16, 0>},   {256, 32, 0, kb_track<256, 32, 0>},
-    {512, 4, 0, kb_track<512, 4, 0>},     {512, 8, 0, kb_track<512, 8, 0>},     {512, 16, 0, kb_track<512, 16, 0>},
-    {512, 32, 0, kb_track<512, 32, 0>},   {1024, 4, 0, kb_track<1024, 4, 0>},   {1024, 8, 0, kb_track<1024, 8, 0>},
-    {1024, 16, 0, kb_track<1024, 16, 0>}, {1024, 32, 0, kb_track<1024, 32, 0>},
+    {256, 8, 0, 1, kb_track<256, 8, 0>},     {256, 16, 0, 1, kb_track<256, 16, 0>},
+    {256, 32, 0, 1, kb_track<256, 32, 0>},   {512, 4, 0, 1, kb_track<512, 4, 0>},
+    {512, 8, 0, 1, kb_track<512, 8, 0>},     {512, 16, 0, 1, kb_track<512, 16, 0>},
+    {512, 32, 0, 1, kb_track<512, 32, 0>},   {1024, 4, 0, 1, kb_track<1024, 4, 0>},
+    {1024, 8, 0, 1, kb_track<1024, 8, 0>},   {1024, 16, 0, 1, kb_track<1024, 16, 0>},
+    {1024, 32, 0, 1, kb_track<1024, 32, 0>},
     // RZ variants: ppt is the unroll depth of a loop over the pair's points
-    {256, 2, 1, kb_track<256, 2, 1>},     {256, 4, 1, kb_track<256, 4, 1>},     {512, 2, 1, kb_track<512, 2, 1>},
-    {512, 4, 1, kb_track<512, 4, 1>},     {1024, 2, 1, kb_track<1024, 2, 1>},
+    {256, 2, 1, 1, kb_track<256, 2, 1>},     {256, 4, 1, 1, kb_track<256, 4, 1>},
+    {512, 2, 1, 1, kb_track<512, 2, 1>},     {512, 4, 1, 1, kb_track<512, 4, 1>},
+    {1024, 2, 1, 1, kb_track<1024, 2, 1>},
 };
 
 }  // namespace rt
@@ -799,8 +807,8 @@ static int autotune(rt_batch* b) {
     b->tuned[key] = {bv, bg};
     b->variant = bv; b->grid = bg;
     if (getenv("RT_VERBOSE"))
-        fprintf(stderr, "rt_batch autotune %s -> bs %d ppt %d rz %d grid %d (%.3f ms)\n", key.c_str(), kVariants[bv].bs,
-                kVariants[bv].ppt, kVariants[bv].rz, bg, best);
+        fprintf(stderr, "rt_batch autotune %s -> bs %d ppt %d rz %d wpe %d grid %d (%.3f ms)\n", key.c_str(),
+                kVariants[bv].bs, kVariants[bv].ppt, kVariants[bv].rz, kVariants[bv].wpe, bg, best);
     return RT_OK;
 }
 
