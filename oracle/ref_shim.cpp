@@ -14,4 +14,5 @@ void ref_mat_mult4_d(double* A, double* B, double* C) { matrixMult4x4(A, B, C); 
 void ref_rot2quat_f(float* m, float* q) { rot2Quaternion(m, 4, q); }
 void ref_normalize_quat_f(float* q) { normalizeQuaternion(q); }
 void ref_pose_distance(float* dT, double* dist, double* angle) { poseDistance(dT, dist, angle); }
+int ref_inverse3x3_f(float* m, float* mi) { return inverse3x3(m, mi); }
 }

@@ -42,7 +42,7 @@ class RtCnnConfig(ctypes.Structure):
                 ("trans_scale_mm", c_float)]
 
 
-# name -> (restype, argtypes); the exported symbols of include/rt_tracker.h and include/rt_cnn.h
+# name -> (restype, argtypes); the exported symbols of include/rt_*.h
 SIGNATURES = {
     "rt_default_params": (None, [P(RtParams)]),
     "rt_calib_pinhole": (c_int, [c_float, c_float, c_float, c_float, c_float, c_float, P(c_float)]),
@@ -105,6 +105,13 @@ SIGNATURES = {
     "rt_cnn_target_to_pose": (c_int, [P(RtCnnConfig), c_int, P(c_float), P(c_double)]),
     "rt_batch_prepare": (c_int, [c_void_p, c_int, P(RtPair)]),
     "rt_batch_propose": (c_int, [c_void_p, c_void_p]),
+    # include/rt_sensor.h
+    "rt_calib_load_xml": (c_int, [ctypes.c_char_p, c_int, c_int, P(c_float), P(c_int)]),
+    "rt_calib_parse_xml": (c_int, [ctypes.c_char_p, c_size_t, c_int, c_int, P(c_float), P(c_int)]),
+    "rt_pnm_info": (c_int, [ctypes.c_char_p, P(c_int), P(c_int), P(c_int), P(c_int)]),
+    "rt_pnm_read": (c_int, [ctypes.c_char_p, c_void_p, c_size_t]),
+    "rt_disparity_to_depth": (c_int, [c_void_p, c_int, c_int, P(c_float), c_int, c_int, c_int, c_void_p, c_void_p]),
+    "rt_depth_mm_to_u16": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p]),
 }
 
 GEOMETRIC, PHOTOMETRIC, BIOBJECTIVE = 0, 1, 2

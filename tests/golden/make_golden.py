@@ -4,7 +4,8 @@ Runs in the build container only: it loads oracle/_ref/libref.so, which
 oracle/Makefile (target `ref`) compiles from the reference sources
 tracker/tracker/expm.c (EXPOKIT Pade expm, expm.c:2017-2090) and
 tracker/tracker/basic_math.cpp (invertRT4, rot2Quaternion, normalizeQuaternion,
-matrixMult4x4, poseDistance). The fixture holds inputs and outputs only.
+matrixMult4x4, poseDistance, inverse3x3). The fixture holds inputs and outputs
+only.
 """
 import ctypes
 import os
@@ -105,6 +106,46 @@ def main():
         ref.ref_normalize_quat_f(q.ctypes.data_as(f))
         QN[i] = q
     out["quat_R"], out["quat_q"], out["quat_qn"] = Rs, Q, QN
+
+    # inverse3x3 (float): intrinsics blocks as the calibration file gives them
+    # at several widths, generic matrices, and nearly singular ones (identity
+    # branch)
+    rng2 = np.random.default_rng(2017)
+    K = np.zeros((24, 9), np.float32)
+    Ki = np.zeros((24, 9), np.float32)
+    for i in range(24):
+        if i < 12:
+            w = [160, 320, 640, 1280][i % 4]
+            fx, fy = rng2.uniform(0.7, 1.0), rng2.uniform(0.7, 1.0)
+            cx, cy = rng2.uniform(0.45, 0.55), rng2.uniform(0.35, 0.4)
+            k = np.array([-fx * w, 0, cx * w, 0, fy * w, cy * w, 0, 0, 1])
+        elif i < 20:
+            k = rng2.normal(size=9)
+        else:
+            k = rng2.normal(size=9) * 1e-2
+        K[i] = k.astype(np.float32)
+        a = K[i].copy(); b = np.zeros(9, np.float32)
+        ref.ref_inverse3x3_f(a.ctypes.data_as(f), b.ctypes.data_as(f))
+        Ki[i] = b
+    out["inv3_K"], out["inv3_Ki"] = K, Ki
+
+    # poseDistance: translation length and the largest absolute Euler angle of
+    # relative transforms (basic_math.cpp:1059-1094)
+    P = np.zeros((32, 16), np.float32)
+    PD = np.zeros((32, 2))
+    for i in range(32):
+        ax = rng2.normal(size=3); ax /= np.linalg.norm(ax)
+        ang = rng2.uniform(0, [0.05, 0.5, 2.0, 3.1][i % 4])
+        Kx = np.array([[0, -ax[2], ax[1]], [ax[2], 0, -ax[0]], [-ax[1], ax[0], 0]])
+        R = np.eye(3) + np.sin(ang) * Kx + (1 - np.cos(ang)) * Kx @ Kx
+        T = np.eye(4); T[:3, :3] = R; T[:3, 3] = rng2.normal(0, [1.0, 50.0, 500.0, 2000.0][i % 4], 3)
+        if i == 0:
+            T = np.eye(4)
+        P[i] = T.astype(np.float32).ravel()
+        a = P[i].copy(); dd = ctypes.c_double(); aa = ctypes.c_double()
+        ref.ref_pose_distance(a.ctypes.data_as(f), ctypes.byref(dd), ctypes.byref(aa))
+        PD[i] = dd.value, aa.value
+    out["pd_T"], out["pd_dist_angle"] = P, PD
 
     np.savez(os.path.join(HERE, "ref_pose_math.npz"), **out)
     print("wrote", os.path.join(HERE, "ref_pose_math.npz"), {k: v.shape for k, v in out.items()})

@@ -2,6 +2,7 @@
 header, parameter and calibration helpers, argument validation, the seeded
 synthetic generator, and the oracle tracker end to end."""
 import ctypes
+import glob
 import os
 import re
 
@@ -17,8 +18,8 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def test_library_exports_every_declared_symbol():
     names = set()
-    for h in ("rt_tracker.h", "rt_cnn.h"):
-        hdr = open(os.path.join(ROOT, "include", h)).read()
+    for h in sorted(glob.glob(os.path.join(ROOT, "include", "rt_*.h"))):
+        hdr = open(h).read()
         hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
         names |= set(re.findall(r"\b(rt_[a-z0-9_]+)\s*\(", hdr))
     assert len(names) >= 55
