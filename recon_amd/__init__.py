@@ -112,6 +112,15 @@ SIGNATURES = {
     "rt_pnm_read": (c_int, [ctypes.c_char_p, c_void_p, c_size_t]),
     "rt_disparity_to_depth": (c_int, [c_void_p, c_int, c_int, P(c_float), c_int, c_int, c_int, c_void_p, c_void_p]),
     "rt_depth_mm_to_u16": (c_int, [c_void_p, c_size_t, c_void_p, c_void_p]),
+    # include/rt_trajectory.h
+    "rt_trajectory_write": (c_int, [ctypes.c_char_p, c_int, P(c_float), c_int, c_int]),
+    "rt_trajectory_read": (c_int, [ctypes.c_char_p, c_int, c_int, c_int, P(c_float), P(c_int)]),
+    "rt_schedule_read": (c_int, [ctypes.c_char_p, c_int, c_void_p, P(c_int)]),
+    "rt_schedule_lookup": (c_int, [c_void_p, c_int, c_int, P(c_int), P(c_float)]),
+    "rt_pose_distance": (c_int, [P(c_float), P(c_double), P(c_double)]),
+    "rt_grid_object": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float, c_float, P(c_float)]),
+    "rt_keyframes_similar": (c_int, [P(c_float), c_int, P(c_float), c_float, c_float, P(c_float), P(c_float), c_int,
+                                     P(c_int), P(c_float), P(c_int)]),
 }
 
 GEOMETRIC, PHOTOMETRIC, BIOBJECTIVE = 0, 1, 2

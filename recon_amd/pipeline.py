@@ -7,9 +7,11 @@ tracker (bi-objective by default) runs Gauss-Newton from that proposal. The
 plain method stays fallback and judge: when the frame's weighted residual RMS
 exceeds mean + reloc_sigma * std of the recent frames, the frame is also
 tracked from the plain start (the previous estimate) and the lower RMS wins.
-If the RMS still spikes, the frame is tracked against the most recent
-keyframes (up to reloc_keyframes) in one batched launch from learned
-proposals; the best few are re-tracked by a second single-pair tracker with the
+If the RMS still spikes, the frame is tracked against up to reloc_keyframes
+keyframes in one batched launch from learned proposals: first those the
+reference's keyframe search finds near the current estimate (pose distance
+within reloc_search = (mm, degrees), ranked by the image displacement of a
+test grid, KeyFrameModel::findSimilarKeyFrames), then the most recent ones; the best few are re-tracked by a second single-pair tracker with the
 same settings (so that their RMS compares with the frame's), and tracking moves
 to the best of them when it fits better and agrees with the current estimate
 within reloc_gate (degrees, mm). The first frames after a keyframe are fused
@@ -30,6 +32,7 @@ from . import BIOBJECTIVE, PHOTOMETRIC
 from .cnn import pair_array
 from .hip import DeviceBuffer, hip
 from .tracker import BatchTracker, PhotoTracker
+from .trajectory import grid_object, similar_keyframes
 
 
 def _angle_dist(T):
@@ -45,7 +48,8 @@ def _exceeded(T, dist_mm, angle_deg):
 class SequenceTracker:
     def __init__(self, width, height, calib, net=None, mode=BIOBJECTIVE, lam=0.5, n_select=61440, fuse_frames=5,
                  max_interval=10, dist_mm=60.0, angle_deg=5.0, relocalise=True, reloc_keyframes=64, reloc_sigma=3.0,
-                 reloc_window=30, reloc_points=8192, reloc_verify=3, reloc_gate=(20.0, 300.0)):
+                 reloc_window=30, reloc_points=8192, reloc_verify=3, reloc_gate=(20.0, 300.0),
+                 reloc_search=(1000.0, 45.0), reloc_history=1024):
         self.W, self.H = width, height
         self.calib = calib
         self.net = net
@@ -55,6 +59,8 @@ class SequenceTracker:
         self.relocalise = relocalise and net is not None
         self.reloc_keyframes, self.reloc_sigma, self.reloc_window = reloc_keyframes, reloc_sigma, reloc_window
         self.reloc_verify, self.reloc_gate = reloc_verify, reloc_gate
+        self.reloc_search, self.reloc_history = reloc_search, reloc_history
+        self.grid = grid_object()
         self.tr = PhotoTracker(width, height, calib, mode=mode, n_select=n_select, lam=lam)
         if net is not None:
             self.x = DeviceBuffer(2 * net.input_elems)
@@ -92,12 +98,27 @@ class SequenceTracker:
             tr.track_from(T_init)
         return tr.get_relative_pose().astype(np.float64), tr.residual_rms()
 
+    def _candidates(self, keyframes, G_est):
+        """Keyframes to relocalise against: those near G_est by the keyframe
+        search, most similar first, then the most recent others."""
+        kfs = list(keyframes)
+        poses = np.array([G for (_, G) in kfs], np.float32)
+        near, _ = similar_keyframes(G_est.astype(np.float32), poses, self.reloc_search[0], self.reloc_search[1],
+                                    self.calib[:9], self.grid)
+        chosen = [int(i) for i in near[:self.reloc_keyframes]]
+        for i in range(len(kfs) - 1, -1, -1):
+            if len(chosen) >= self.reloc_keyframes:
+                break
+            if i not in chosen:
+                chosen.append(i)
+        return [kfs[i] for i in chosen]
+
     def _relocalise(self, g, d, k, keyframes, G_est):
         """Track frame k against the recent keyframes in one batched launch, then
         re-track the best reloc_verify of them with the second single-pair
         tracker; returns (keyframe index, its global pose, pose relative to it,
         rms) of the best one that agrees with the estimate G_est, or None."""
-        cand = list(keyframes)[-self.reloc_keyframes:]
+        cand = self._candidates(keyframes, G_est)
         cg, cd = self._frame(g, d, k)
         pairs = []
         for (kf, _) in cand:
@@ -132,7 +153,7 @@ class SequenceTracker:
         poses[0] = np.eye(4)
         self._set_keyframe(g, d, 0)
         kf, G_kf = 0, np.eye(4)
-        keyframes = collections.deque([(0, np.eye(4))], maxlen=self.reloc_keyframes)
+        keyframes = collections.deque([(0, np.eye(4))], maxlen=self.reloc_history)
         hist = collections.deque(maxlen=self.reloc_window)
         n_kf, n_reloc = 1, 0
         trace = []  # per frame: (keyframe, rms, 0 first estimate / 1 plain fallback / 2 relocalised)
