@@ -20,6 +20,7 @@
 #include "../../include/rt_cnn.h"
 #include "cnn.hpp"
 #include "rt_host.hpp"
+#include "rt_stage.hpp"
 
 namespace rt {
 
@@ -563,6 +564,7 @@ struct rt_cnn {
     int device = 0, max_batch = 0;
     rt_cnn_config cfg;
     hipStream_t stream = nullptr;
+    rt::HostStage st_src, st_flags;  // pinned staging of the frame table and the augmentation flags
     int H[7], W[7];
     size_t w_off[6], b_off[6], fc_w, fc_b, nparams;
     int K[6], Kp[6];
@@ -698,6 +700,8 @@ extern "C" void rt_cnn_destroy(rt_cnn* net) {
     if (!net) return;
     hipSetDevice(net->device);
     hipStreamSynchronize(net->stream);
+    net->st_src.release();
+    net->st_flags.release();
     for (void* p : net->allocs) hipFree(p);
     hipStreamDestroy(net->stream);
     delete net;
@@ -855,7 +859,7 @@ extern "C" int rt_cnn_augment(rt_cnn* net, int n, const uint16_t* x_dev, const f
     if (rc) return rc;
     if (!y_dev || !y_inv_dev || !flags || !x_out || !y_out) return set_error(RT_ERR_ARG, "rt_cnn_augment: null argument");
     if (net->cfg.channels[0] != 4) return set_error(RT_ERR_UNSUPPORTED, "rt_cnn_augment: 4-channel inputs only");
-    if (hipMemcpyAsync(net->flags_dev, flags, 4 * (size_t)n, hipMemcpyHostToDevice, net->stream) != hipSuccess)
+    if (!net->st_flags.upload(net->flags_dev, flags, 4 * (size_t)n, net->stream))
         return set_error(RT_ERR_HIP, "rt_cnn_augment: flags copy");
     int cells = net->cfg.in_width * net->cfg.in_height;
     k_augment<<<dim3((cells + 255) / 256, n), 256, 0, net->stream>>>(x_dev, y_dev, y_inv_dev, net->flags_dev, net->cfg.in_width,
@@ -870,7 +874,7 @@ static int make_inputs(rt_cnn* net, int n, const FrameSrc* src, bf16* x, hipStre
         if (f.W < net->cfg.in_width || f.H < net->cfg.in_height || f.W % net->cfg.in_width || f.H % net->cfg.in_height)
             return set_error(RT_ERR_ARG, "rt_cnn: frame size must be a multiple of the network input size");
     }
-    if (hipMemcpyAsync(net->src_dev, src, sizeof(FrameSrc) * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess)
+    if (!net->st_src.upload(net->src_dev, src, sizeof(FrameSrc) * (size_t)n, s))
         return set_error(RT_ERR_HIP, "rt_cnn: frame table copy");
     int cells = net->cfg.in_width * net->cfg.in_height;
     k_inputs<<<dim3((cells + 255) / 256, n), 256, 0, s>>>(net->src_dev, net->cfg.in_width, net->cfg.in_height, x);

@@ -27,6 +27,7 @@
 #include "rt_kernels.hpp"
 #include "rt_prep.hpp"
 #include "rt_host.hpp"
+#include "rt_stage.hpp"
 
 namespace rt {
 
@@ -644,7 +645,9 @@ struct rt_batch {
     float* trel_out = nullptr;
     float* rms = nullptr;
     int* status = nullptr;
-    float* trel_in_host = nullptr;   // pinned staging
+    // pinned staging of the host tables each prepare / retrack uploads (the
+    // copies are queued, so their sources must outlive the call)
+    HostStage st_desc, st_calib, st_order, st_trel;
     // last batch
     int n = 0;
     std::vector<PairDesc> desc;
@@ -693,7 +696,7 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
               dal((void**)&b->queue, 256) && dal((void**)&b->calib_dev, 96 * 4 * (size_t)max_pairs) &&
               dal((void**)&b->trel_in, 64 * (size_t)max_pairs) && dal((void**)&b->trel_out, 64 * (size_t)max_pairs) &&
               dal((void**)&b->rms, 4 * (size_t)max_pairs) && dal((void**)&b->status, 4 * (size_t)max_pairs);
-    if (!ok || hipHostMalloc((void**)&b->trel_in_host, 64 * (size_t)max_pairs, 0) != hipSuccess) {
+    if (!ok) {
         for (void* p : b->allocs) hipFree(p);
         hipStreamDestroy(b->stream);
         delete b;
@@ -710,7 +713,7 @@ extern "C" void rt_batch_destroy(rt_batch* b) {
     if (b->graph_prep) hipGraphExecDestroy(b->graph_prep);
     if (b->graph_track) hipGraphExecDestroy(b->graph_track);
     for (void* p : b->allocs) hipFree(p);
-    if (b->trel_in_host) hipHostFree(b->trel_in_host);
+    b->st_desc.release(); b->st_calib.release(); b->st_order.release(); b->st_trel.release();
     hipStreamDestroy(b->stream);
     delete b;
 }
@@ -865,6 +868,7 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
     if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
     std::vector<PairDesc> desc(n);
     std::vector<float> calib(96 * (size_t)n);
+    std::vector<float> trel(16 * (size_t)n);
     int maxW = 0, maxH = 0, maxN = 0;
     for (int i = 0; i < n; i++) {
         const rt_pair& p = pairs[i];
@@ -907,7 +911,7 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
         d.trel_out = b->trel_out + 16 * (size_t)i;
         d.rms = b->rms + i;
         d.status = b->status + i;
-        for (int k = 0; k < 16; k++) b->trel_in_host[16 * (size_t)i + k] = p.init_T ? (float)p.init_T[k] : ((k % 5 == 0) ? 1.0f : 0.0f);
+        for (int k = 0; k < 16; k++) trel[16 * (size_t)i + k] = p.init_T ? (float)p.init_T[k] : ((k % 5 == 0) ? 1.0f : 0.0f);
         maxW = std::max(maxW, d.W); maxH = std::max(maxH, d.H); maxN = std::max(maxN, d.N);
     }
     // work queue order: decreasing cost (points x iterations), ties by index
@@ -915,10 +919,10 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
     for (int i = 0; i < n; i++) order[i] = i;
     auto cost = [&](int i) { long c = 0; for (int k = 0; k < desc[i].L; k++) c += desc[i].iters[k]; return c * desc[i].N + (long)desc[i].W * desc[i].H / 8; };
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
-    if (hipMemcpyAsync(b->desc_dev, desc.data(), sizeof(PairDesc) * n, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-        hipMemcpyAsync(b->calib_dev, calib.data(), calib.size() * 4, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-        hipMemcpyAsync(b->order_dev, order.data(), 4 * (size_t)n, hipMemcpyHostToDevice, b->stream) != hipSuccess ||
-        hipMemcpyAsync(b->trel_in, b->trel_in_host, 64 * (size_t)n, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+    if (!b->st_desc.upload(b->desc_dev, desc.data(), sizeof(PairDesc) * n, b->stream) ||
+        !b->st_calib.upload(b->calib_dev, calib.data(), calib.size() * 4, b->stream) ||
+        !b->st_order.upload(b->order_dev, order.data(), 4 * (size_t)n, b->stream) ||
+        !b->st_trel.upload(b->trel_in, trel.data(), 64 * (size_t)n, b->stream))
         return set_error(RT_ERR_HIP, "descriptor upload");
     bool changed = b->n != n || b->maxW != maxW || b->maxH != maxH || b->maxN != maxN;
     b->n = n; b->desc = desc; b->maxW = maxW; b->maxH = maxH; b->maxN = maxN;
@@ -979,9 +983,9 @@ extern "C" int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_resul
     if (!b || b->n < 1) return set_error(RT_ERR_STATE, "rt_batch_retrack: no prepared batch");
     if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
     if (init_T) {
-        for (int i = 0; i < b->n; i++)
-            for (int k = 0; k < 16; k++) b->trel_in_host[16 * (size_t)i + k] = (float)init_T[16 * (size_t)i + k];
-        if (hipMemcpyAsync(b->trel_in, b->trel_in_host, 64 * (size_t)b->n, hipMemcpyHostToDevice, b->stream) != hipSuccess)
+        std::vector<float> trel(16 * (size_t)b->n);
+        for (size_t i = 0; i < trel.size(); i++) trel[i] = (float)init_T[i];
+        if (!b->st_trel.upload(b->trel_in, trel.data(), 64 * (size_t)b->n, b->stream))
             return set_error(RT_ERR_HIP, "pose upload");
     }
     return run_track(b, out);

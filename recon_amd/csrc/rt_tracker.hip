@@ -212,6 +212,7 @@ static int upload_frame(rt_tracker* t, int slot, const uint8_t* g, const uint16_
     if (on_device) { *g_dev = g; *d_dev = dd; return RT_OK; }
     HIP_TRY(hipMemcpyAsync(t->g8[slot], g, npx, hipMemcpyHostToDevice, t->stream));
     HIP_TRY(hipMemcpyAsync(t->d16[slot], dd, npx * 2, hipMemcpyHostToDevice, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));  // host frames are read during the call only
     *g_dev = t->g8[slot]; *d_dev = t->d16[slot];
     return RT_OK;
 }
@@ -234,6 +235,7 @@ extern "C" int rt_set_current_rgbd(rt_tracker* t, const float* rgbd, int on_devi
     const float* src = rgbd;
     if (!on_device) {
         HIP_TRY(hipMemcpyAsync(t->rgbd, rgbd, (size_t)t->W * t->H * 16, hipMemcpyHostToDevice, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));  // host frames are read during the call only
         src = t->rgbd;
     }
     launch_convert_pyramid(t->d, t->d.cur, nullptr, nullptr, src, true, false, t->stream);
@@ -321,6 +323,7 @@ extern "C" int rt_set_reference_rgbd(rt_tracker* t, const float* rgbd, int on_de
     const float* src = rgbd;
     if (!on_device) {
         HIP_TRY(hipMemcpyAsync(t->rgbd, rgbd, (size_t)t->W * t->H * 16, hipMemcpyHostToDevice, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));  // host frames are read during the call only
         src = t->rgbd;
     }
     return set_reference_impl(t, nullptr, nullptr, src, suggested_mode, lambda, force);
@@ -629,6 +632,7 @@ extern "C" int rt_track_sequence(rt_tracker* t, const uint8_t* gray, const uint1
     double pose0[32]; ident(pose0); ident(pose0 + 16);
     HIP_TRY(hipMemcpyAsync(t->d.pose, pose0, sizeof(pose0), hipMemcpyHostToDevice, t->stream));
     HIP_TRY(hipMemcpyAsync(t->seq_poses, pose0, 16 * sizeof(double), hipMemcpyHostToDevice, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));  // pose0 lives on this function's stack
     int mode = t->mode;
     int rc = set_reference_impl(t, gray, depth, nullptr, mode, t->lambda, 1);
     if (rc < 0) return rc;
