@@ -3,6 +3,7 @@
 // _d: FP64 oracle.
 #include "cnn.hpp"
 #include "tracker.hpp"
+#include "tsdf.hpp"
 
 using namespace oracle;
 
@@ -183,6 +184,28 @@ double or_cnn_forward_backward(const int* ch, int H0, int W0, int B, const float
 }
 void or_bf16_round(const float* x, int n, float* out) {
     for (int i = 0; i < n; i++) out[i] = cnn::bf16_round(x[i]);
+}
+
+// TSDF fusion and ray casting (tsdf.hpp). prm: res, w, h; box: origin[3], dim[3], min, max
+static tsdf::Grid tsdf_grid(const int* prm, const float* box) {
+    tsdf::Grid g;
+    tsdf::make_grid(g, prm[0], box, box + 3, prm[1], prm[2], box[6], box[7]);
+    return g;
+}
+void or_tsdf_dist_rgbd(const int* prm, const float* box, const float* rgbd, float* dist) {
+    tsdf::dist_rgbd(tsdf_grid(prm, box), rgbd, dist);
+}
+void or_tsdf_update(const int* prm, const float* box, float* grid, const float* dist, const float* T, const float* K) {
+    tsdf::update(tsdf_grid(prm, box), grid, dist, T, K);
+}
+void or_tsdf_resample(const int* prm, const float* box, float* dst, const float* src, const float* T) {
+    tsdf::resample(tsdf_grid(prm, box), dst, src, T);
+}
+void or_tsdf_raycast(const int* prm, const float* box, const float* grid, const float* T, const float* K, int refine,
+                     float* xyz, float* normals, float* shade) {
+    const tsdf::Grid g = tsdf_grid(prm, box);
+    tsdf::raycast(g, grid, T, K, refine != 0, xyz);
+    tsdf::shade(g, xyz, shade, normals);
 }
 
 }  // extern "C"

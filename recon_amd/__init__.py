@@ -121,6 +121,20 @@ SIGNATURES = {
     "rt_grid_object": (c_int, [c_int, c_int, c_int, c_float, c_float, c_float, c_float, P(c_float)]),
     "rt_keyframes_similar": (c_int, [P(c_float), c_int, P(c_float), c_float, c_float, P(c_float), P(c_float), c_int,
                                      P(c_int), P(c_float), P(c_int)]),
+    # include/rt_tsdf.h
+    "rt_tsdf_default_params": (None, [c_void_p]),
+    "rt_tsdf_create": (c_int, [c_int, c_void_p, P(c_void_p)]),
+    "rt_tsdf_destroy": (None, [c_void_p]),
+    "rt_tsdf_reset": (c_int, [c_void_p]),
+    "rt_tsdf_set_depth_rgbd": (c_int, [c_void_p, c_void_p]),
+    "rt_tsdf_set_depth_u16": (c_int, [c_void_p, c_void_p]),
+    "rt_tsdf_update": (c_int, [c_void_p, P(c_float), P(c_float)]),
+    "rt_tsdf_raycast": (c_int, [c_void_p, P(c_float), P(c_float), c_int, c_void_p, c_void_p, c_void_p]),
+    "rt_tsdf_resample": (c_int, [c_void_p, c_void_p, P(c_float)]),
+    "rt_tsdf_read": (c_int, [c_void_p, c_void_p, c_size_t]),
+    "rt_tsdf_data": (c_void_p, [c_void_p]),
+    "rt_tsdf_stream": (c_void_p, [c_void_p]),
+    "rt_tsdf_synchronize": (c_int, [c_void_p]),
 }
 
 GEOMETRIC, PHOTOMETRIC, BIOBJECTIVE = 0, 1, 2

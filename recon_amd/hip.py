@@ -60,6 +60,13 @@ class DeviceBuffer:
         _ok(hip().hipMemcpy(b.ptr, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
         return b
 
+    def upload(self, a):
+        """Copy a host array into the start of the buffer (synchronous)."""
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError(f"upload of {a.nbytes} bytes into a {self.nbytes}-byte buffer")
+        _ok(hip().hipMemcpy(self.ptr, a.ctypes.data, a.nbytes, 1), "hipMemcpy H2D")
+
     def to_array(self, dtype, count):
         out = np.empty(count, dtype)
         _ok(hip().hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2), "hipMemcpy D2H")
