@@ -13,6 +13,10 @@
 #include "rt_host.hpp"
 #include "rt_tsdf.h"
 
+#ifndef TSDF_AHEAD
+#define TSDF_AHEAD 8  // voxels of a ray whose loads are in flight together
+#endif
+
 namespace rt {
 
 struct TsdfArgs {
@@ -203,15 +207,52 @@ __global__ __launch_bounds__(256) void k_tsdf_raycast(const float2* __restrict__
             float tnx = (vpx + stx - ox) * ivx, tny = (vpy + sty - oy) * ivy, tnz = (vpz + stz - oz) * ivz;
             const float tsx = stx * ivx, tsy = sty * ivy, tsz = stz * ivz;
             int iter = refine ? 3 : 0;
+            // The walk does not depend on the distances it reads, so TSDF_AHEAD
+            // voxels are stepped and their loads issued together, then tested
+            // in walk order: the same voxels, tests and arithmetic as one
+            // voxel at a time, with the load latency paid once per group.
+            const float* gd = reinterpret_cast<const float*>(g);
             float t = enterT;
-            while (t < exitT) {
-                if (vx < 0 || vy < 0 || vz < 0 || vx >= res || vy >= res || vz >= res) break;  // the walk left the grid
-                const size_t offset = (size_t)vx + (size_t)vy * res + (size_t)vz * res * res;
-                if (g[offset].x < 0.0f) {
-                    float t1 = fminf(fmaxf((vpx - ox) * dx + (vpy - oy) * dy + (vpz - oz) * dz, enterT), exitT);
+            bool done = false;
+            // the grid check ends the walk within 3 * (res + 1) steps: each step moves one index one way
+            while (!done) {
+                float val[TSDF_AHEAD], wpx[TSDF_AHEAD], wpy[TSDF_AHEAD], wpz[TSDF_AHEAD];
+                bool live = true;
+                int nlive = 0;
+                // branch-free, so that the loads are issued back to back: each
+                // reads a voxel clamped into the grid, and a voxel past the end
+                // of the walk is ignored
+#pragma unroll
+                for (int j = 0; j < TSDF_AHEAD; j++) {
+                    live = live && t < exitT && vx >= 0 && vy >= 0 && vz >= 0 && vx < res && vy < res && vz < res;
+                    const size_t offset = (size_t)min(max(vx, 0), res - 1) + (size_t)min(max(vy, 0), res - 1) * res +
+                                          (size_t)min(max(vz, 0), res - 1) * res * res;
+                    val[j] = gd[2 * offset];
+                    nlive += live ? 1 : 0;
+                    wpx[j] = vpx; wpy[j] = vpy; wpz[j] = vpz;
+                    // the step to the nearest voxel wall: x if tnx < tny and tnx < tnz, else y if
+                    // tny <= tnx and tny < tnz, else z
+                    const bool sx = tnx < tny && tnx < tnz, sy = !(tnx < tny) && tny < tnz, sz = !sx && !sy;
+                    t = sx ? tnx : (sy ? tny : tnz);
+                    vpx = sx ? vpx + stx : vpx; vx = sx ? vx + gx : vx; tnx = sx ? tnx + tsx : tnx;
+                    vpy = sy ? vpy + sty : vpy; vy = sy ? vy + gy : vy; tny = sy ? tny + tsy : tny;
+                    vpz = sz ? vpz + stz : vpz; vz = sz ? vz + gz : vz; tnz = sz ? tnz + tsz : tnz;
+                }
+                unsigned cand = 0;
+#pragma unroll
+                for (int j = 0; j < TSDF_AHEAD; j++) cand |= (j < nlive && val[j] < 0.0f) ? (1u << j) : 0u;
+                while (cand && !done) {
+                    const int j = __builtin_ctz(cand);
+                    cand &= cand - 1;
+                    float px = wpx[0], py = wpy[0], pz = wpz[0];
+#pragma unroll
+                    for (int k = 1; k < TSDF_AHEAD; k++)
+                        if (k == j) { px = wpx[k]; py = wpy[k]; pz = wpz[k]; }
+                    float t1 = fminf(fmaxf((px - ox) * dx + (py - oy) * dy + (pz - oz) * dz, enterT), exitT);
                     float e1 = trilinear(g, res, (ox + t1 * dx - ibx) * ivdx, (oy + t1 * dy - iby) * ivdy,
                                          (oz + t1 * dz - ibz) * ivdz);
                     if (e1 < 0.0f) {
+                        done = true;  // found, or the surface was expected and not found
                         float t0 = t1 - vdx / 2;
                         // at most the cube's diagonal in half-voxel steps
                         for (int back = 0; t0 > enterT && back < 4 * res + 16; back++) {
@@ -234,16 +275,9 @@ __global__ __launch_bounds__(256) void k_tsdf_raycast(const float2* __restrict__
                             }
                             t0 -= vdx * 0.5f;
                         }
-                        break;  // found, or the surface was expected and not found
                     }
                 }
-                if (tnx < tny) {
-                    if (tnx < tnz) { t = tnx; vpx += stx; vx += gx; tnx += tsx; }
-                    else { t = tnz; vpz += stz; vz += gz; tnz += tsz; }
-                } else {
-                    if (tny < tnz) { t = tny; vpy += sty; vy += gy; tny += tsy; }
-                    else { t = tnz; vpz += stz; vz += gz; tnz += tsz; }
-                }
+                if (nlive < TSDF_AHEAD) done = true;  // the walk passed the exit or left the grid
             }
         }
         if (hit) result = make_float4(tHit * cx, tHit * cy, tHit * cz, (tHit > a.mn && tHit < a.mx) ? 1.0f : 0.0f);
