@@ -9,8 +9,9 @@
 //  - there is no tracker thread and no barrier (phototracker.cpp:691-728): the
 //    caller's thread runs one frame with track() (readyToTrack, gpuPreProcess,
 //    gpuEstimateIncrement and applyIncrement in one call);
-//  - setReferencePoints takes the measured keyframe (the raw-measurement branch
-//    of shapetracker.cpp:195-211); raycast point inputs are not part of this path.
+//  - setReferencePoints takes the keyframe frame and, for the filtered
+//    reference, the ray-cast point and normal maps as device float4 images
+//    (the TSDF model of rt_tsdf.h makes them).
 // Returned pose pointers are borrowed arrays owned by the object, valid until
 // the next call that changes the pose, as in the reference. Failures print
 // rt_last_error() to stderr and return false / leave state unchanged; the
@@ -81,6 +82,23 @@ public:
         if (rc < 0) report("setReferencePoints");
         return rc == 1;
     }
+    // with the ray-cast maps of a filtered reference (device float4 images)
+    bool setReferencePoints(const float* xyz, const float* normals, const uint8_t* gray, const uint16_t* depth,
+                            bool onDevice, TrackMode suggestedMethod, float lambda = 0.0f, bool force = false) {
+        int rc = rt_set_reference_points(h_, xyz, normals, gray, depth, onDevice ? 1 : 0, suggestedMethod, lambda,
+                                         force ? 1 : 0);
+        if (rc < 0) report("setReferencePoints");
+        return rc == 1;
+    }
+    bool setReferencePoints(const float* xyz, const float* normals, const float* rgbd, bool onDevice,
+                            TrackMode suggestedMethod, float lambda = 0.0f, bool force = false) {
+        int rc = rt_set_reference_points_rgbd(h_, xyz, normals, rgbd, onDevice ? 1 : 0, suggestedMethod, lambda,
+                                              force ? 1 : 0);
+        if (rc < 0) report("setReferencePoints");
+        return rc == 1;
+    }
+    void setFilteredReference(bool flag) { check(rt_set_filtered_reference(h_, flag ? 1 : 0), "setFilteredReference"); }
+    bool getFilteredReference() const { return rt_get_filtered_reference(h_) == 1; }
     // One frame of the tracker thread's loop body (phototracker.cpp:704-718);
     // false when not ready to track (no keyframe or no current frame).
     bool track() {

@@ -111,6 +111,29 @@ int rt_set_reference(rt_tracker* t, const uint8_t* gray, const uint16_t* depth, 
 int rt_set_reference_rgbd(rt_tracker* t, const float* rgbd, int on_device, int suggested_mode, float lambda,
                           int force);
 
+/* PhotoTracker::setFilteredReference (phototracker.cpp:71-74,
+ * shapetracker.cpp:59-65): with the flag on, the ICP keyframe comes from
+ * ray-cast points and normals (the TSDF model, include/rt_tsdf.h) instead of
+ * the measured depth; the photometric keyframe is the frame either way. Off
+ * by default, as in the reference application (rgbdscan/src/main.cpp:1236).
+ * While on, rt_set_reference / rt_set_reference_rgbd fail (the reference's
+ * setReferencePoints returns false without point maps, phototracker.cpp:313)
+ * and so does rt_track_sequence, whose keyframes are measured frames. */
+int rt_set_filtered_reference(rt_tracker* t, int flag);
+int rt_get_filtered_reference(const rt_tracker* t);  /* getFilteredReference :82-84 */
+/* PhotoTracker::setReferencePoints(xyz, normals, frame, mode, lambda)
+ * (phototracker.cpp:310-337) with the ray-cast maps: xyz and normals are
+ * device float4 images of width x height (camera coordinates of the
+ * keyframe; xyz.w and normals.w are not read), e.g. rt_tsdf_raycast's outputs.
+ * With the filtered reference on, the ICP cloud keeps the points with z < 0
+ * and |n|^2 > 1e-7 (generateOrientedPoints6Cuda, d2z.cu:471-502,
+ * 1169-1181); with it off, the maps are ignored and the call is
+ * rt_set_reference / rt_set_reference_rgbd. Return values as there. */
+int rt_set_reference_points(rt_tracker* t, const float* xyz, const float* normals, const uint8_t* gray,
+                            const uint16_t* depth, int on_device, int suggested_mode, float lambda, int force);
+int rt_set_reference_points_rgbd(rt_tracker* t, const float* xyz, const float* normals, const float* rgbd,
+                                 int on_device, int suggested_mode, float lambda, int force);
+
 /* readyToTrack -> gpuPreProcess -> gpuEstimateIncrement -> applyIncrement
  * (phototracker.cpp:704-718). Returns 1 when a frame was tracked, 0 when not
  * ready (no keyframe or no current frame for the mode), < 0 on error. */

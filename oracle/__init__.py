@@ -38,6 +38,7 @@ def lib():
                 getattr(_lib, f"or_tracker_{name}_{s}").argtypes = [ctypes.c_void_p]
             getattr(_lib, f"or_tracker_set_reference_{s}").argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
             getattr(_lib, f"or_tracker_set_current_{s}").argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+            getattr(_lib, f"or_tracker_set_reference_points_{s}").argtypes = [ctypes.c_void_p] * 5
             getattr(_lib, f"or_tracker_set_mode_{s}").argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_float]
             getattr(_lib, f"or_tracker_set_trel_{s}").argtypes = [ctypes.c_void_p, ctypes.c_void_p]
             getattr(_lib, f"or_tracker_get_{s}").argtypes = [ctypes.c_void_p] + [ctypes.c_void_p] * 4
@@ -92,6 +93,13 @@ class Tracker:
     def set_reference(self, gray, depth):
         g = np.ascontiguousarray(gray, np.uint8); d = np.ascontiguousarray(depth, np.uint16)
         self._f("set_reference")(self.h, ptr(g), ptr(d))
+
+    def set_reference_points(self, gray, depth, xyz, normals):
+        """Keyframe of a filtered reference: photometric from the frame, ICP
+        cloud from ray-cast float4 maps (h, w, 4)."""
+        g = np.ascontiguousarray(gray, np.uint8); d = np.ascontiguousarray(depth, np.uint16)
+        x = np.ascontiguousarray(xyz, np.float32); n = np.ascontiguousarray(normals, np.float32)
+        self._f("set_reference_points")(self.h, ptr(g), ptr(d), ptr(x), ptr(n))
 
     def set_current(self, gray, depth):
         g = np.ascontiguousarray(gray, np.uint8); d = np.ascontiguousarray(depth, np.uint16)

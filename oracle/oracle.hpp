@@ -760,6 +760,25 @@ void ref_cloud(const R* z, int W, int H, const float* calib, R* pts, int* mask) 
         }
 }
 
+// genOrientedPoints6Kernel (tracker/kernels/d2z.cu:471-502): the keyframe
+// cloud of a filtered reference from ray-cast points and normals (float4
+// images); kept where z < 0 and |n|^2 > 1e-7 (FP32 test, as in the kernel),
+// every other record zero.
+template <typename R>
+void oriented_cloud(const float* xyz, const float* nrm, int W, int H, R* pts, int* mask) {
+    for (size_t o = 0; o < (size_t)W * H; o++) {
+        const float* v = xyz + 4 * o;
+        const float* n = nrm + 4 * o;
+        const float len = n[0] * n[0] + n[1] * n[1] + n[2] * n[2];
+        const bool keep = v[2] < 0 && len > 1e-7f;
+        for (int k = 0; k < 3; k++) {
+            pts[o * 6 + k] = keep ? (R)v[k] : (R)0;
+            pts[o * 6 + 3 + k] = keep ? (R)n[k] : (R)0;
+        }
+        mask[o] = keep ? 1 : 0;
+    }
+}
+
 // ---------------------------------------------------------------- A19
 // genPointsCloud6Kernel (tracker/kernels/d2z.cu:436-469): every z < 0 is valid,
 // normal := (0,0,1).

@@ -98,6 +98,9 @@ void or_rot2quat(const float* m, float* q, float* qn) {
     for (int i = 0; i < 4; i++) qn[i] = q[i] / len;
 }
 void or_invert_rt4_d(const double* M, double* Mi) { invert_rt4<double>(M, Mi); }
+void or_oriented_cloud_f(const float* xyz, const float* nrm, int W, int H, float* pts, int* mask) {
+    oriented_cloud<float>(xyz, nrm, W, H, pts, mask);
+}
 void or_invert_rt4_f(const float* M, float* Mi) { invert_rt4<float>(M, Mi); }
 void or_mat4_mul(const double* A, const double* B, double* C) { mat4_mul(A, B, C); }
 int or_pose_difference_exceeded(const float* dT, float tthr, float athr) {
@@ -127,6 +130,10 @@ static Params make_params(const int* p, float lambda) {
     void or_tracker_destroy_##S(void* h) { delete (Tr_##S*)h; }                                      \
     void or_tracker_set_reference_##S(void* h, const uint8_t* g, const uint16_t* d) {                \
         ((Tr_##S*)h)->set_reference_u8(g, d);                                                        \
+    }                                                                                                \
+    void or_tracker_set_reference_points_##S(void* h, const uint8_t* g, const uint16_t* d, const float* xyz, \
+                                             const float* nrm) {                                     \
+        ((Tr_##S*)h)->set_reference_points_u8(g, d, xyz, nrm);                                       \
     }                                                                                                \
     void or_tracker_set_current_##S(void* h, const uint8_t* g, const uint16_t* d) {                  \
         ((Tr_##S*)h)->set_current_u8(g, d);                                                          \

@@ -118,6 +118,14 @@ struct Tracker {
         pending_pose_reset = true;
         set_icp_reference();
     }
+    // setReferencePoints with ray-cast maps and the filtered reference on
+    // (phototracker.cpp:310-337, shapetracker.cpp:199-200): the photometric
+    // keyframe from the frame, the ICP cloud from the maps
+    void set_reference_points_u8(const uint8_t* g, const uint16_t* d, const float* xyz, const float* nrm) {
+        set_reference_u8(g, d);
+        ref_cloud_[0].assign((size_t)W * H * 6, 0); ref_mask[0].assign((size_t)W * H, 0);
+        oriented_cloud<R>(xyz, nrm, W, H, ref_cloud_[0].data(), ref_mask[0].data());
+    }
     // ShapeTracker::setReferencePoints, raw-measurement branch (shapetracker.cpp:195-211)
     void set_icp_reference() {
         ref_cloud_[0].assign((size_t)W * H * 6, 0); ref_mask[0].assign((size_t)W * H, 0);

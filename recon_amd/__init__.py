@@ -52,6 +52,10 @@ SIGNATURES = {
     "rt_set_current_rgbd": (c_int, [c_void_p, c_void_p, c_int]),
     "rt_set_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_int]),
     "rt_set_reference_rgbd": (c_int, [c_void_p, c_void_p, c_int, c_int, c_float, c_int]),
+    "rt_set_filtered_reference": (c_int, [c_void_p, c_int]),
+    "rt_get_filtered_reference": (c_int, [c_void_p]),
+    "rt_set_reference_points": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_int]),
+    "rt_set_reference_points_rgbd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_int]),
     "rt_track": (c_int, [c_void_p]),
     "rt_get_global_pose": (c_int, [c_void_p, P(c_double)]),
     "rt_track_from": (c_int, [c_void_p, P(c_double)]),

@@ -358,6 +358,24 @@ void launch_ref_cloud(const Dev& d, hipStream_t s) {
     dim3 g16((d.W + 15) / 16, (d.H + 15) / 16);
     k_ref_cloud<<<g16, 256, 0, s>>>(d.ref.z, d.W, d.H, d.calib, d.rpos[0], d.rnrm[0]);
 }
+
+// genOrientedPoints6Kernel (d2z.cu:471-502): the keyframe cloud from ray-cast
+// points and normals; a point is kept where z < 0 and |n|^2 > 1e-7, every
+// other record is zero.
+__global__ __launch_bounds__(256) void k_ref_oriented(const float4* __restrict__ xyz, const float4* __restrict__ nrm,
+                                                      int n, float4* __restrict__ pos, float4* __restrict__ rn) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float4 v = xyz[i], q = nrm[i];
+    const float len = q.x * q.x + q.y * q.y + q.z * q.z;
+    const bool keep = v.z < 0 && len > 1e-7f;
+    pos[i] = keep ? make_float4(v.x, v.y, v.z, 1.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    rn[i] = keep ? make_float4(q.x, q.y, q.z, 0.0f) : make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+void launch_ref_oriented(const Dev& d, const float* xyz, const float* nrm, hipStream_t s) {
+    const int n = d.W * d.H;
+    k_ref_oriented<<<(n + 255) / 256, 256, 0, s>>>((const float4*)xyz, (const float4*)nrm, n, d.rpos[0], d.rnrm[0]);
+}
 void launch_ref_cloud_pyramid(const Dev& d, hipStream_t s) {
     for (int l = 1; l < d.L; l++) {
         int n = (d.W >> l) * (d.H >> l);

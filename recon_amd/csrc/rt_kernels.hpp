@@ -80,6 +80,7 @@ void launch_reset_pose(float* T, double* pose, hipStream_t s);
 void launch_fill(float* p, float v, int n, hipStream_t s);
 void launch_prepare_photo_reference(const Dev& d, hipStream_t s);
 void launch_ref_cloud(const Dev& d, hipStream_t s);
+void launch_ref_oriented(const Dev& d, const float* xyz, const float* nrm, hipStream_t s);
 void launch_ref_cloud_pyramid(const Dev& d, hipStream_t s);
 void launch_cur_cloud(const Dev& d, hipStream_t s);
 void launch_photo_iteration(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s);

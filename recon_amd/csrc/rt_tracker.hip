@@ -54,6 +54,8 @@ struct rt_tracker {
     float lambda = 0.9f;
     bool referenceExists = false, currentExists = false, referencePrepared = false;
     bool icpReferenceExists = false, icpPrepared = false;
+    bool filteredReference = false;  // ICP keyframe from ray-cast points (setFilteredReference)
+    bool icpFromPoints = false;      // the current ICP keyframe came from point maps
     bool pendingPoseReset = false;
     int refFrameNumber = 0, currentFrame = 0, referenceUpdates = 0;
     float T[16];
@@ -260,17 +262,21 @@ static bool icp_outdated(const rt_tracker* t) {
 
 static void set_mode(rt_tracker* t, int mode) { t->mode = mode; }
 
-// updatePhotometricReference / ShapeTracker::setReferencePoints on a converted keyframe
-static int update_reference(rt_tracker* t, const uint8_t* g, const uint16_t* dd, const float* rgbd, bool photo,
-                            bool icp) {
+// updatePhotometricReference / ShapeTracker::setReferencePoints on a converted
+// keyframe: the ICP cloud from the measured depth, or from ray-cast points and
+// normals when the filtered reference is on (shapetracker.cpp:199-207)
+static int update_reference(rt_tracker* t, const uint8_t* g, const uint16_t* dd, const float* rgbd, const float* xyz,
+                            const float* nrm, bool photo, bool icp) {
     launch_convert_pyramid(t->d, t->d.ref, g, dd, rgbd, false, true, t->stream);
     HIP_TRY(hipGetLastError());
     // fusion weights restart at one
     launch_fill(t->d.kf_w, 1.0f, t->W * t->H, t->stream);
     if (photo) { t->referenceExists = true; t->referencePrepared = false; }
     if (icp) {
-        launch_ref_cloud(t->d, t->stream);
+        if (t->filteredReference) launch_ref_oriented(t->d, xyz, nrm, t->stream);
+        else launch_ref_cloud(t->d, t->stream);
         HIP_TRY(hipGetLastError());
+        t->icpFromPoints = t->filteredReference;
         t->icpReferenceExists = true; t->icpPrepared = false;
     }
     t->pendingPoseReset = true;
@@ -279,27 +285,30 @@ static int update_reference(rt_tracker* t, const uint8_t* g, const uint16_t* dd,
     return RT_OK;
 }
 
-static int set_reference_impl(rt_tracker* t, const uint8_t* g, const uint16_t* dd, const float* rgbd, int mode,
-                              float lambda, int force) {
+static int set_reference_impl(rt_tracker* t, const uint8_t* g, const uint16_t* dd, const float* rgbd, const float* xyz,
+                              const float* nrm, int mode, float lambda, int force) {
     if (mode < 0 || mode > 2) return set_error(RT_ERR_ARG, "bad mode");
+    // phototracker.cpp:313: a filtered reference needs the ray-cast maps
+    if (t->filteredReference && (!xyz || !nrm))
+        return set_error(RT_ERR_ARG, "filtered reference: ray-cast points and normals are required");
     int rc;
     if (mode == RT_PHOTOMETRIC) {
         if (force || photo_outdated(t)) {
             set_mode(t, RT_PHOTOMETRIC);
-            if ((rc = update_reference(t, g, dd, rgbd, true, false))) return rc;
+            if ((rc = update_reference(t, g, dd, rgbd, xyz, nrm, true, false))) return rc;
             return 1;
         }
     } else if (mode == RT_GEOMETRIC) {
         if (force || icp_outdated(t)) {
             set_mode(t, RT_GEOMETRIC);
-            if ((rc = update_reference(t, g, dd, rgbd, false, true))) return rc;
+            if ((rc = update_reference(t, g, dd, rgbd, xyz, nrm, false, true))) return rc;
             return 1;
         }
     } else {
         if (force || photo_outdated(t) || icp_outdated(t)) {
             set_mode(t, RT_BIOBJECTIVE);
             t->lambda = lambda < 0 ? 0.0f : (lambda > 1 ? 1.0f : lambda);
-            if ((rc = update_reference(t, g, dd, rgbd, true, true))) return rc;
+            if ((rc = update_reference(t, g, dd, rgbd, xyz, nrm, true, true))) return rc;
             return 1;
         }
     }
@@ -313,7 +322,7 @@ extern "C" int rt_set_reference(rt_tracker* t, const uint8_t* gray, const uint16
     const uint8_t* g; const uint16_t* dd;
     int rc = upload_frame(t, 0, gray, depth, on_device, &g, &dd);
     if (rc) return rc;
-    return set_reference_impl(t, g, dd, nullptr, suggested_mode, lambda, force);
+    return set_reference_impl(t, g, dd, nullptr, nullptr, nullptr, suggested_mode, lambda, force);
 }
 
 extern "C" int rt_set_reference_rgbd(rt_tracker* t, const float* rgbd, int on_device, int suggested_mode,
@@ -326,7 +335,41 @@ extern "C" int rt_set_reference_rgbd(rt_tracker* t, const float* rgbd, int on_de
         HIP_TRY(hipStreamSynchronize(t->stream));  // host frames are read during the call only
         src = t->rgbd;
     }
-    return set_reference_impl(t, nullptr, nullptr, src, suggested_mode, lambda, force);
+    return set_reference_impl(t, nullptr, nullptr, src, nullptr, nullptr, suggested_mode, lambda, force);
+}
+
+extern "C" int rt_set_reference_points(rt_tracker* t, const float* xyz, const float* normals, const uint8_t* gray,
+                                       const uint16_t* depth, int on_device, int suggested_mode, float lambda,
+                                       int force) {
+    if (!t) return set_error(RT_ERR_ARG, "null tracker");
+    HIP_TRY(hipSetDevice(t->device));
+    const uint8_t* g; const uint16_t* dd;
+    int rc = upload_frame(t, 0, gray, depth, on_device, &g, &dd);
+    if (rc) return rc;
+    return set_reference_impl(t, g, dd, nullptr, xyz, normals, suggested_mode, lambda, force);
+}
+
+extern "C" int rt_set_reference_points_rgbd(rt_tracker* t, const float* xyz, const float* normals, const float* rgbd,
+                                            int on_device, int suggested_mode, float lambda, int force) {
+    if (!t || !rgbd) return set_error(RT_ERR_ARG, "null argument");
+    HIP_TRY(hipSetDevice(t->device));
+    const float* src = rgbd;
+    if (!on_device) {
+        HIP_TRY(hipMemcpyAsync(t->rgbd, rgbd, (size_t)t->W * t->H * 16, hipMemcpyHostToDevice, t->stream));
+        HIP_TRY(hipStreamSynchronize(t->stream));  // host frames are read during the call only
+        src = t->rgbd;
+    }
+    return set_reference_impl(t, nullptr, nullptr, src, xyz, normals, suggested_mode, lambda, force);
+}
+
+extern "C" int rt_set_filtered_reference(rt_tracker* t, int flag) {
+    if (!t) return set_error(RT_ERR_ARG, "null tracker");
+    t->filteredReference = flag != 0;
+    return RT_OK;
+}
+extern "C" int rt_get_filtered_reference(const rt_tracker* t) {
+    if (!t) return set_error(RT_ERR_ARG, "null tracker");
+    return t->filteredReference ? 1 : 0;
 }
 
 static bool ready(const rt_tracker* t) {
@@ -588,7 +631,7 @@ extern "C" int rt_fuse_current(rt_tracker* t) {
 // Re-prepare the keyframe from its (fused) level-0 images, keeping the pose.
 static int refresh_keyframe(rt_tracker* t) {
     launch_convert_pyramid(t->d, t->d.ref, nullptr, nullptr, nullptr, false, true, t->stream);
-    if (t->icpReferenceExists) {
+    if (t->icpReferenceExists && !t->icpFromPoints) {  // a cloud from ray-cast points stays as it is
         launch_ref_cloud(t->d, t->stream);
         t->icpPrepared = false;
     }
@@ -615,6 +658,8 @@ extern "C" int rt_track_sequence(rt_tracker* t, const uint8_t* gray, const uint1
     if (!t || !gray || !depth || !poses_out || n < 1) return set_error(RT_ERR_ARG, "rt_track_sequence: bad argument");
     if (interval < 1) return set_error(RT_ERR_ARG, "keyframe_interval must be >= 1");
     if (fuse_frames < 0 || fuse_frames >= interval) return set_error(RT_ERR_ARG, "fuse_frames must be in [0, interval)");
+    if (t->filteredReference)
+        return set_error(RT_ERR_STATE, "rt_track_sequence keys on measured frames: turn the filtered reference off");
     HIP_TRY(hipSetDevice(t->device));
     if (t->seq_cap < n) {
         if (t->seq_poses) {
@@ -634,12 +679,13 @@ extern "C" int rt_track_sequence(rt_tracker* t, const uint8_t* gray, const uint1
     HIP_TRY(hipMemcpyAsync(t->seq_poses, pose0, 16 * sizeof(double), hipMemcpyHostToDevice, t->stream));
     HIP_TRY(hipStreamSynchronize(t->stream));  // pose0 lives on this function's stack
     int mode = t->mode;
-    int rc = set_reference_impl(t, gray, depth, nullptr, mode, t->lambda, 1);
+    int rc = set_reference_impl(t, gray, depth, nullptr, nullptr, nullptr, mode, t->lambda, 1);
     if (rc < 0) return rc;
     int kf = 0;
     for (int k = 1; k < n; k++) {
         if (k - 1 > 0 && (k - 1) % interval == 0) {
-            rc = set_reference_impl(t, gray + (k - 1) * npx, depth + (k - 1) * npx, nullptr, mode, t->lambda, 1);
+            rc = set_reference_impl(t, gray + (k - 1) * npx, depth + (k - 1) * npx, nullptr, nullptr, nullptr, mode,
+                                    t->lambda, 1);
             if (rc < 0) return rc;
             kf = k - 1;
         }

@@ -79,6 +79,26 @@ class PhotoTracker:
         self.L.rt_synchronize(self.h)
         return bool(r)
 
+    def set_reference_from_points(self, xyz, normals, gray, depth, mode=PHOTOMETRIC, lam=0.0, force=False,
+                                  on_device=False):
+        """setReferencePoints with ray-cast maps: xyz and normals are device
+        pointers to float4 images of the tracker's size (e.g. a VoxelGrid's
+        ray cast); the frame (gray, depth) is host or device data."""
+        if on_device:
+            return bool(check(self.L.rt_set_reference_points(self.h, xyz, normals, gray, depth, 1, mode, lam, int(force)),
+                              "rt_set_reference_points"))
+        g = np.ascontiguousarray(gray, np.uint8); d = np.ascontiguousarray(depth, np.uint16)
+        r = check(self.L.rt_set_reference_points(self.h, xyz, normals, g.ctypes.data, d.ctypes.data, 0, mode, lam,
+                                                 int(force)), "rt_set_reference_points")
+        self.L.rt_synchronize(self.h)
+        return bool(r)
+
+    def set_filtered_reference(self, flag):
+        check(self.L.rt_set_filtered_reference(self.h, int(flag)), "rt_set_filtered_reference")
+
+    def get_filtered_reference(self):
+        return bool(check(self.L.rt_get_filtered_reference(self.h), "rt_get_filtered_reference"))
+
     def set_reference_rgbd(self, rgbd, mode=PHOTOMETRIC, lam=0.0, force=False):
         a = _f32(rgbd)
         r = check(self.L.rt_set_reference_rgbd(self.h, a.ctypes.data, 0, mode, lam, int(force)), "rt_set_reference_rgbd")

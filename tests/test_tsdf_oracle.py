@@ -248,3 +248,27 @@ def test_tsdf_create_rejects_bad_params():
     q = default_params()
     assert q.resolution == 256 and q.width == 320 and q.height == 240
     assert abs(q.origin[2] + 1600.01) < 1e-3 and q.dim[0] == 1500.0
+
+
+def test_oriented_cloud_keep_rule():
+    """The filtered reference's ICP cloud (genOrientedPoints6Kernel): a point
+    is kept where z < 0 and |n|^2 > 1e-7, and its record is the point and the
+    normal; everything else is zero."""
+    import oracle
+    W, H = 16, 8
+    rng = np.random.default_rng(5)
+    xyz = rng.normal(0, 100, (H, W, 4)).astype(np.float32)
+    nrm = rng.normal(0, 1, (H, W, 4)).astype(np.float32)
+    nrm[0, :4, :3] = [[0, 0, 0], [3e-4, 0, 0], [2e-4, 2e-4, 2e-4], [0, 0, -1]]
+    xyz[0, :4, 2] = -5.0
+    pts = np.zeros(W * H * 6, np.float32)
+    mask = np.zeros(W * H, np.int32)
+    oracle.call("or_oriented_cloud_f", xyz, nrm, W, H, pts, mask)
+    n2 = (nrm[..., 0] * nrm[..., 0] + nrm[..., 1] * nrm[..., 1] + nrm[..., 2] * nrm[..., 2]).ravel()
+    keep = (xyz[..., 2].ravel() < 0) & (n2 > np.float32(1e-7))
+    assert np.array_equal(mask.astype(bool), keep)
+    assert list(keep[:4]) == [False, False, True, True]  # 9e-8 < 1e-7 < 1.2e-7
+    p = pts.reshape(-1, 6)
+    assert np.array_equal(p[keep, :3], xyz.reshape(-1, 4)[keep, :3])
+    assert np.array_equal(p[keep, 3:], nrm.reshape(-1, 4)[keep, :3])
+    assert np.all(p[~keep] == 0)
