@@ -593,12 +593,15 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
                     sys[t] = v;
                 }
                 __syncthreads();
-                if (t == 0) {
+                if (wave == 0) {
                     SolveCfg sc;
                     sc.solver = cfg.solver; sc.mode = 1; sc.lambda = 0.0f; sc.scale_out = cfg.scale_out;
                     sc.robust = cfg.robust; sc.mad_bins = cfg.mad_bins; sc.scale_in = 1e-3f;
-                    if (!solve_update(sys, T, sc, N, 1, ws)) s_ok = 0;
-                    rms_last = sys[28] > 0.0 ? (float)sqrt(sys[27] / sys[28]) : 0.0f;
+                    const bool ok = solve_update_wave(sys, T, sc, N, ws);
+                    if (t == 0) {
+                        if (!ok) s_ok = 0;
+                        rms_last = sys[28] > 0.0 ? (float)sqrt(sys[27] / sys[28]) : 0.0f;
+                    }
                 }
                 __syncthreads();
             }

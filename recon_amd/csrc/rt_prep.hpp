@@ -481,6 +481,72 @@ __device__ bool solve_update(const double* sys, float* trel, const SolveCfg& cfg
     return ok;
 }
 
+// One double of lane `src` (a compile-time constant after unrolling) to every lane.
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
+    return __hiloint2double(hi, lo);
+}
+// solve_update for a photometric system with the Cholesky solver, run by the
+// 64 lanes of one wave: lane r < 6 holds row r of A and of L in registers, the
+// factorisation computes a column's entries of all rows at once, and the
+// substitutions exchange values by lane reads instead of a workspace. Every
+// entry is computed by the same operations in the same order as in
+// solve_cholesky, so the result is the same bit for bit; the exponential
+// update runs on lane 0 as in solve_update. Other solvers and modes run
+// solve_update on lane 0. The return value is meaningful in lane 0.
+__device__ bool solve_update_wave(const double* sys, float* trel, const SolveCfg& cfg, int N1, double* w) {
+    const int lane = lane_id();
+    if (cfg.solver != 0 || cfg.mode != 1) {
+        bool ok = true;
+        if (lane == 0) ok = solve_update(sys, trel, cfg, N1, 1, w);
+        return ok;
+    }
+    const int r = lane < 6 ? lane : 5;
+    // row r of A and b_r from the 27 packed terms (unpack27's order)
+    double Ar[6];
+#pragma unroll
+    for (int c = 0; c < 6; c++) {
+        const int lo = r < c ? r : c, hi = r < c ? c : r;
+        Ar[c] = sys[lo * 6 - lo * (lo - 1) / 2 + (hi - lo)];
+    }
+    const double br = sys[21 + r];
+    double Lr[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+        // lane j: s = A[j][j] - sum L[j][k]^2; lanes i > j: t = A[i][j] - sum L[i][k] L[j][k]
+        double u = Ar[j];
+#pragma unroll
+        for (int k = 0; k < j; k++) u -= Lr[k] * lane_bcast(Lr[k], j);
+        const double sj = lane_bcast(u, j);
+        if (!(sj > 0.0)) { ok = false; break; }
+        const double dg = sqrt(sj);
+        const double q = u / dg;
+        Lr[j] = r == j ? dg : (r > j ? q : Lr[j]);
+    }
+    double x[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    if (ok) {
+        double yv = 0.0;
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double t = br;
+#pragma unroll
+            for (int k = 0; k < i; k++) t -= Lr[k] * lane_bcast(yv, k);
+            const double yi = t / Lr[i];
+            yv = r == i ? yi : yv;
+        }
+#pragma unroll
+        for (int i = 5; i >= 0; i--) {
+            double t = lane_bcast(yv, i);
+#pragma unroll
+            for (int k = i + 1; k < 6; k++) t -= lane_bcast(Lr[i], k) * x[k];
+            x[i] = t / lane_bcast(Lr[i], i);
+        }
+    }
+    if (lane == 0) expm_update(x, trel, (double)cfg.scale_out, w + WS_EXPM);
+    return ok;
+}
 
 // ------------------------------------------------------------------ A10 + A11 per point
 // interpolateResidualKernel2 (warp.cu:269-332) for one point: r = I_ref(l) -
