@@ -481,6 +481,87 @@ __device__ bool solve_update(const double* sys, float* trel, const SolveCfg& cfg
     return ok;
 }
 
+// expm_update with every matrix in registers instead of a workspace: the same
+// operations in the same order (se3_exp, normalize_mat4_safe, the product
+// T * Ti, normalize_mat4_safe). Run by every lane of a wave on the same x,
+// so that no lane waits on the others; lane `writer` stores the pose.
+__device__ __forceinline__ void normalize_mat4_safe_regs(double* T) {
+    double u[3] = {T[0], T[4], T[8]}, v[3] = {T[1], T[5], T[9]}, w[3] = {T[2], T[6], T[10]};
+    double lenU = (double)sqrtf((float)(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]));
+    double lenW = (double)sqrtf((float)(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]));
+    if (lenW > 1e-8f) { w[0] /= lenW; w[1] /= lenW; w[2] /= lenW; }
+    else { w[0] = 0; w[1] = 0; w[2] = 1; }
+    u[0] = v[1] * w[2] - v[2] * w[1];
+    u[1] = -(v[0] * w[2] - v[2] * w[0]);
+    u[2] = v[0] * w[1] - v[1] * w[0];
+    if (lenU > 1e-8f) { u[0] /= lenU; u[1] /= lenU; u[2] /= lenU; }
+    else { u[0] = 1; u[1] = 0; u[2] = 0; }
+    v[0] = w[1] * u[2] - w[2] * u[1];
+    v[1] = -(w[0] * u[2] - w[2] * u[0]);
+    v[2] = w[0] * u[1] - w[1] * u[0];
+    T[0] = u[0]; T[4] = u[1]; T[8] = u[2];
+    T[1] = v[0]; T[5] = v[1]; T[9] = v[2];
+    T[2] = w[0]; T[6] = w[1]; T[10] = w[2];
+}
+__device__ __forceinline__ void expm_update_regs(const double* x, float* TF, double scale, bool writer) {
+    double Ti[16];
+    double mx = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) mx = fmax(mx, fabs(x[i]));
+    if (mx != 0) {
+        // se3_exp
+        const double wx = x[0], wy = x[1], wz = x[2];
+        const double th2 = wx * wx + wy * wy + wz * wz, th = sqrt(th2);
+        double a, b, cc;
+        if (th < 1e-4) {
+            a = 1.0 - th2 / 6.0 + th2 * th2 / 120.0;
+            b = 0.5 - th2 / 24.0 + th2 * th2 / 720.0;
+            cc = 1.0 / 6.0 - th2 / 120.0 + th2 * th2 / 5040.0;
+        } else {
+            const double sn = sin(th), co = cos(th);
+            a = sn / th; b = (1.0 - co) / th2; cc = (th - sn) / (th2 * th);
+        }
+        const double Wm[9] = {0, -wz, wy, wz, 0, -wx, -wy, wx, 0};
+        double W2[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                double s2 = 0;
+#pragma unroll
+                for (int k = 0; k < 3; k++) s2 += Wm[i * 3 + k] * Wm[k * 3 + j];
+                W2[i * 3 + j] = s2;
+            }
+        double V[9];
+#pragma unroll
+        for (int i = 0; i < 9; i++) {
+            const double I = (i % 4 == 0) ? 1.0 : 0.0;
+            Ti[(i / 3) * 4 + (i % 3)] = I + a * Wm[i] + b * W2[i];
+            V[i] = I + b * Wm[i] + cc * W2[i];
+        }
+#pragma unroll
+        for (int i = 0; i < 3; i++) Ti[i * 4 + 3] = V[i * 3 + 0] * x[3] + V[i * 3 + 1] * x[4] + V[i * 3 + 2] * x[5];
+        Ti[12] = 0; Ti[13] = 0; Ti[14] = 0; Ti[15] = 1;
+        normalize_mat4_safe_regs(Ti);
+        Ti[3] *= scale; Ti[7] *= scale; Ti[11] *= scale;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) Ti[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    }
+    double R[16];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const double t0 = TF[i * 4 + 0], t1 = TF[i * 4 + 1], t2 = TF[i * 4 + 2], t3 = TF[i * 4 + 3];
+#pragma unroll
+        for (int j = 0; j < 4; j++) R[i * 4 + j] = t0 * Ti[0 * 4 + j] + t1 * Ti[1 * 4 + j] + t2 * Ti[2 * 4 + j] + t3 * Ti[3 * 4 + j];
+    }
+    normalize_mat4_safe_regs(R);
+    if (writer) {
+#pragma unroll
+        for (int i = 0; i < 16; i++) TF[i] = (float)R[i];
+    }
+}
+
 // One double of lane `src` (a compile-time constant after unrolling) to every lane.
 __device__ __forceinline__ double lane_bcast(double v, int src) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(v), src);
@@ -544,7 +625,7 @@ __device__ bool solve_update_wave(const double* sys, float* trel, const SolveCfg
             x[i] = t / lane_bcast(Lr[i], i);
         }
     }
-    if (lane == 0) expm_update(x, trel, (double)cfg.scale_out, w + WS_EXPM);
+    expm_update_regs(x, trel, (double)cfg.scale_out, lane == 0);
     return ok;
 }
 
