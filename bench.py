@@ -307,13 +307,17 @@ def main():
     ok_b, rb, tb = conv(Trel_b, judge)
     ok_i, _, _ = conv(ident, judge)
     ok_p, rp, tp = conv(np.array(plain_T), judge[:P])
-    ra, ta = pose_errors(ident[:P], plain_T)
+    # the batched and the plain tracker sum the normal equations in different
+    # orders, so they agree to rounding where both converge; a diverging pair
+    # amplifies the difference, so the comparison is over the pairs both converge on
+    both = ok_i[:P] & ok_p
+    ra, ta = pose_errors(ident[:P][both], np.array(plain_T)[both]) if both.any() else (np.zeros(1), np.zeros(1))
     rtb, ttb = pose_errors(Trel_b, gt)
     log(f"{'learned+batched' if learned else 'batched'}: {B} pairs, {step_s * 1e3:.2f} ms/step -> {value:.0f} frames/s "
         f"({value / plain_fps:.1f}x plain); kb_track {k_ms:.2f} ms, {achieved:.0f} GB/s algorithmic")
     log(f"converged (<=0.1 deg / 1 mm of the judge): {'learned+batched' if learned else 'batched'} {ok_b.mean() * 100:.1f}%, "
         f"batched from identity {ok_i.mean() * 100:.1f}%, plain from identity {ok_p.mean() * 100:.1f}% (of {P}); "
-        f"batched vs plain from identity max {ra.max():.2e} deg {ta.max():.2e} mm")
+        f"batched vs plain from identity, {int(both.sum())} pairs both converge on: max {ra.max():.2e} deg {ta.max():.2e} mm")
 
     # ---- FP64 oracle from the true motion on a sample: error of the converged results
     fp64 = None
@@ -359,6 +363,7 @@ def main():
         "pose_error": {"converged_median_deg_vs_truth": float(np.median(rtb[ok_b])) if ok_b.any() else None,
                        "converged_median_mm_vs_truth": float(np.median(ttb[ok_b])) if ok_b.any() else None,
                        "judge_median_deg_vs_truth": float(np.median(rj)), "judge_median_mm_vs_truth": float(np.median(tj)),
+                       "batched_vs_plain_identity_pairs": int(both.sum()),
                        "batched_vs_plain_identity_max_deg": float(ra.max()),
                        "batched_vs_plain_identity_max_mm": float(ta.max()), "vs_fp64_oracle": fp64},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
