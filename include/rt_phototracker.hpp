@@ -97,6 +97,18 @@ public:
         if (rc < 0) report("setReferencePoints");
         return rc == 1;
     }
+    // updateErrorImage / getDebugImage1C / getRefImage1C, copied to host memory
+    void updateErrorImage() { check(rt_update_error_image(h_), "updateErrorImage"); }
+    bool getDebugImage1C(int layer, float* out, size_t count) {
+        int rc = rt_get_debug_image(h_, layer, out, count);
+        if (rc < 0) report("getDebugImage1C");
+        return rc == RT_OK;
+    }
+    bool getRefImage1C(int layer, float* out, size_t count) {
+        int rc = rt_get_ref_image(h_, layer, out, count);
+        if (rc < 0) report("getRefImage1C");
+        return rc == RT_OK;
+    }
     void setFilteredReference(bool flag) { check(rt_set_filtered_reference(h_, flag ? 1 : 0), "setFilteredReference"); }
     bool getFilteredReference() const { return rt_get_filtered_reference(h_) == 1; }
     // One frame of the tracker thread's loop body (phototracker.cpp:704-718);

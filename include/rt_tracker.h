@@ -120,6 +120,23 @@ int rt_set_reference_rgbd(rt_tracker* t, const float* rgbd, int on_device, int s
  * setReferencePoints returns false without point maps, phototracker.cpp:313)
  * and so does rt_track_sequence, whose keyframes are measured frames. */
 int rt_set_filtered_reference(rt_tracker* t, int flag);
+/* Inspection images for display. updateErrorImage (phototracker.cpp:388-403):
+ * photometric mode writes min(25 |r|, 1) of each selected point's level-0
+ * residual under the current pose at its pixel (other pixels keep their
+ * values; xyz2DiffKernel, warp.cu:1955-1999); geometric and bi-objective
+ * modes write, per level, min(|z' - z_cur| / 30, 1) against the current point
+ * the keyframe point projects to, 0 elsewhere (xyz6DiffNormalizedKernel,
+ * rgb2gray.cu:143-171). The keyframe must be prepared (after rt_track or
+ * rt_debug_prepare). */
+int rt_update_error_image(rt_tracker* t);
+/* getDebugImage1C (:734-744) and getRefImage1C (:746-755), copied to host
+ * memory of `count` floats: photometric mode the W x H error image and the
+ * keyframe's depth clamped to [min, max] and mapped to [0, 1] (xyz2Depth-
+ * NormalizedKernel, rgb2gray.cu:104-116); the other modes the level's
+ * geometric error image and the shape tracker's reference image, which the
+ * reference never writes (zeros). */
+int rt_get_debug_image(rt_tracker* t, int level, float* out, size_t count);
+int rt_get_ref_image(rt_tracker* t, int level, float* out, size_t count);
 int rt_get_filtered_reference(const rt_tracker* t);  /* getFilteredReference :82-84 */
 /* PhotoTracker::setReferencePoints(xyz, normals, frame, mode, lambda)
  * (phototracker.cpp:310-337) with the ray-cast maps: xyz and normals are

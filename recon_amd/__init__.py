@@ -53,6 +53,9 @@ SIGNATURES = {
     "rt_set_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_int]),
     "rt_set_reference_rgbd": (c_int, [c_void_p, c_void_p, c_int, c_int, c_float, c_int]),
     "rt_set_filtered_reference": (c_int, [c_void_p, c_int]),
+    "rt_update_error_image": (c_int, [c_void_p]),
+    "rt_get_debug_image": (c_int, [c_void_p, c_int, c_void_p, c_size_t]),
+    "rt_get_ref_image": (c_int, [c_void_p, c_int, c_void_p, c_size_t]),
     "rt_get_filtered_reference": (c_int, [c_void_p]),
     "rt_set_reference_points": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_int]),
     "rt_set_reference_points_rgbd": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_float, c_int]),

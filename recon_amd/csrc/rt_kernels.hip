@@ -132,8 +132,25 @@ __global__ __launch_bounds__(256) void k_ref_points(const float* __restrict__ z,
     ref_point(i, F32Src{z}, F32Src{g0}, g1, g2, g3, W, H, L, N, c, scale_in, sel, pts, j0, j1, j2, j3);
 }
 
+// The keyframe's depth for display: xyz2DepthNormalizedKernel (rgb2gray.cu:104-116)
+// over the points of convertZMapToXYZKernel (:570-593), whose invalid pixels
+// hold z = 0: clamp(-z, min, max) mapped to [0, 1].
+__global__ __launch_bounds__(256) void k_dbg_ref_depth(const float* __restrict__ z, int n, const float* __restrict__ c,
+                                                       float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float mn = c[MIND], mx = c[MAXD];
+    const float zi = z[i];
+    const float pz = (zi < -mn && zi > -mx) ? zi : 0.0f;
+    float zval = -pz;
+    if (zval < mn) zval = mn;
+    if (zval > mx) zval = mx;
+    out[i] = (zval - mn) / (mx - mn);
+}
+
 void launch_prepare_photo_reference(const Dev& d, hipStream_t s) {
     dim3 g16((d.W + 15) / 16, (d.H + 15) / 16);
+    k_dbg_ref_depth<<<(d.W * d.H + 255) / 256, 256, 0, s>>>(d.ref.z, d.W * d.H, d.calib, d.dbg_ref);
     k_ref_features<<<g16, 256, 0, s>>>(d.ref.z, d.ref.gray[0], d.W, d.H, d.calib, d.gmag, d.ghist);
     k_select_count<<<512, 256, 0, s>>>(d.gmag, d.W * d.H, d.ghist, d.N, d.seg);
     k_select_scan<<<1, 512, 0, s>>>(d.seg);
@@ -357,6 +374,66 @@ __global__ __launch_bounds__(256) void k_cloud_down(const float4* __restrict__ s
 void launch_ref_cloud(const Dev& d, hipStream_t s) {
     dim3 g16((d.W + 15) / 16, (d.H + 15) / 16);
     k_ref_cloud<<<g16, 256, 0, s>>>(d.ref.z, d.W, d.H, d.calib, d.rpos[0], d.rnrm[0]);
+}
+
+// xyz2DiffKernel (warp.cu:1955-1999) at level 0: at each selected pixel,
+// min(25 |r|, 1) of its photometric residual under the current pose (1 when
+// it leaves the view); other pixels keep their values.
+__global__ __launch_bounds__(256) void k_dbg_err_photo(const int* __restrict__ sel, const float4* __restrict__ pts,
+                                                       const float4* __restrict__ jac0, int N,
+                                                       const float* __restrict__ img, const float* __restrict__ zcur,
+                                                       int W, int H, const float* __restrict__ c,
+                                                       const float* __restrict__ trel, float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= N) return;
+    float T[16];
+#pragma unroll
+    for (int k = 0; k < 16; k++) T[k] = trel[k];
+    const JacView jv = jac_view(const_cast<float4*>(jac0), N);
+    float ir, zW;
+    photo_point(pts[i], jv.iref[i], T, 1.0f, 0.0f, F32Src{img}, W, H, F32Src{zcur}, W, H, c, ir, zW);
+    const int v = sel[i];
+    if (v >= 0 && v < W * H) out[v] = fminf(fabsf(ir) * 25.0f, 1.0f);
+}
+// xyz6DiffNormalizedKernel (rgb2gray.cu:143-171): per keyframe pixel of a
+// level, min(|z' - z_cur| / 30, 1) against the current point at the nearest
+// pixel its transform projects to (depth intrinsics KL), 0 where either is
+// not selected or it leaves the view.
+__global__ __launch_bounds__(256) void k_dbg_err_icp(const float4* __restrict__ rpos, const float4* __restrict__ cpos,
+                                                     int w, int h, const float* __restrict__ c,
+                                                     const float* __restrict__ trel, float a, float b,
+                                                     float* __restrict__ out) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= w * h) return;
+    const float4 r = rpos[i];
+    float res = 0.0f;
+    if (r.w != 0.0f) {
+        float T[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) T[k] = trel[k];
+        const float* K = c + KL;
+        const f3 q3 = mul4(T, f3{r.x, r.y, r.z});
+        const float qx = q3.x / q3.z, qy = q3.y / q3.z;
+        const float u = a * (K[0] * qx + K[2]) + b, v = a * (K[4] * qy + K[5]) + b;
+        if (u >= 0 && v >= 0 && u < w - 1 && v < h - 1) {
+            const int off2 = (int)(u + 0.5f) + (int)(v + 0.5f) * w;
+            const float4 q = cpos[off2];
+            if (q.w > 0.0f) res = fminf(fabsf(q3.z - q.z) / 30.0f, 1.0f);
+        }
+    }
+    out[i] = res;
+}
+void launch_error_image_photo(const Dev& d, hipStream_t s) {
+    k_dbg_err_photo<<<(d.N + 255) / 256, 256, 0, s>>>(d.sel, d.pts, d.jac[0], d.N, d.cur.gray[0], d.cur.z, d.W, d.H,
+                                                       d.calib, d.trel, d.dbg_err);
+}
+void launch_error_image_icp(const Dev& d, hipStream_t s) {
+    for (int l = 0; l < d.L; l++) {
+        const int w = d.W >> l, h = d.H >> l;
+        const float a = 1.0f / (float)(1 << l), b = 0.5f * (a - 1.0f);
+        k_dbg_err_icp<<<(w * h + 255) / 256, 256, 0, s>>>(d.rpos[l], d.cpos[l], w, h, d.calib, d.trel, a, b,
+                                                           d.dbg_icp[l]);
+    }
 }
 
 // genOrientedPoints6Kernel (d2z.cu:471-502): the keyframe cloud from ray-cast

@@ -62,6 +62,11 @@ struct Dev {
     int* icp_n;                 // 4
     float* bsum;                // Student-t block sums (W*H/1024)
     double* pose;               // device-side pose state for sequences: Tprev(16 float as double) | global(16)
+    // inspection images (updateErrorImage, getDebugImage1C, getRefImage1C)
+    float* dbg_err;             // W*H photometric error image
+    float* dbg_ref;             // W*H normalised keyframe depth
+    float* dbg_icp[4];          // per level geometric error images
+    float* dbg_icp_ref[4];      // per level geometric reference images (never written, as in the reference)
 };
 
 struct SolveCfg {
@@ -87,6 +92,8 @@ void launch_photo_iteration(const Dev& d, int layer, const SolveCfg& c, bool sol
 void launch_icp_select(const Dev& d, int layer, hipStream_t s);
 void launch_icp_system(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s);
 void launch_fuse(const Dev& d, hipStream_t s);
+void launch_error_image_photo(const Dev& d, hipStream_t s);
+void launch_error_image_icp(const Dev& d, hipStream_t s);
 void launch_apply_increment(const Dev& d, double* pose_out, hipStream_t s);
 int photo_groups(int N);
 int icp_groups(int npix);

@@ -168,6 +168,11 @@ extern "C" int rt_create(int device, int W, int H, const float* calib, const rt_
     }
     A(d.rfull, npx); A(d.icp_cnt, G1024); A(d.icp_bits, (size_t)G1024 * 16); A(d.icp_var, 16); A(d.icp_n, 4);
     A(d.bsum, G1024); A(d.pose, 32);
+    A(d.dbg_err, npx); A(d.dbg_ref, npx);
+    for (int l = 0; l < L; l++) {
+        size_t nl = (size_t)(W >> l) * (H >> l);
+        A(d.dbg_icp[l], nl); A(d.dbg_icp_ref[l], nl);
+    }
     for (int k = 0; k < 2; k++) { A(t->g8[k], npx); A(t->d16[k], npx); }
     A(t->rgbd, 4 * npx);
 #undef A
@@ -360,6 +365,51 @@ extern "C" int rt_set_reference_points_rgbd(rt_tracker* t, const float* xyz, con
         src = t->rgbd;
     }
     return set_reference_impl(t, nullptr, nullptr, src, xyz, normals, suggested_mode, lambda, force);
+}
+
+// PhotoTracker::updateErrorImage (phototracker.cpp:388-403): the photometric
+// error image at level 0, or the geometric ones of every level
+// (ShapeTracker::updateErrorImage, shapetracker.cpp:237-239), under the
+// current pose of the prepared keyframe and frame.
+extern "C" int rt_update_error_image(rt_tracker* t) {
+    if (!t) return set_error(RT_ERR_ARG, "null tracker");
+    HIP_TRY(hipSetDevice(t->device));
+    if (t->mode == RT_PHOTOMETRIC) {
+        if (!t->referencePrepared) return set_error(RT_ERR_STATE, "rt_update_error_image: keyframe not prepared");
+        launch_error_image_photo(t->d, t->stream);
+    }
+    if (t->mode == RT_GEOMETRIC || t->mode == RT_BIOBJECTIVE) {
+        if (!t->icpPrepared) return set_error(RT_ERR_STATE, "rt_update_error_image: keyframe not prepared");
+        launch_error_image_icp(t->d, t->stream);
+    }
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return RT_OK;
+}
+static int copy_image(rt_tracker* t, const float* src, size_t n, float* out, size_t count) {
+    if (!out || count < n) return set_error(RT_ERR_ARG, "image buffer too small");
+    HIP_TRY(hipSetDevice(t->device));
+    HIP_TRY(hipMemcpyAsync(out, src, n * sizeof(float), hipMemcpyDeviceToHost, t->stream));
+    HIP_TRY(hipStreamSynchronize(t->stream));
+    return RT_OK;
+}
+// getDebugImage1C (phototracker.cpp:734-744): photometric mode the W x H error
+// image whatever the level; the other modes the geometric one of `level`.
+extern "C" int rt_get_debug_image(rt_tracker* t, int level, float* out, size_t count) {
+    if (!t) return set_error(RT_ERR_ARG, "null tracker");
+    if (t->mode == RT_PHOTOMETRIC) return copy_image(t, t->d.dbg_err, (size_t)t->W * t->H, out, count);
+    if (level < 0 || level >= t->L) return set_error(RT_ERR_ARG, "bad level");
+    return copy_image(t, t->d.dbg_icp[level], (size_t)(t->W >> level) * (t->H >> level), out, count);
+}
+// getRefImage1C (phototracker.cpp:746-755): photometric mode the keyframe's
+// normalised depth (W x H); the other modes the shape tracker's reference
+// image of `level`, which the reference never writes (its normalising kernel,
+// xyz6ToDepthNormalizedKernel, rgb2gray.cu:125-131, stores nothing): zeros.
+extern "C" int rt_get_ref_image(rt_tracker* t, int level, float* out, size_t count) {
+    if (!t) return set_error(RT_ERR_ARG, "null tracker");
+    if (t->mode == RT_PHOTOMETRIC) return copy_image(t, t->d.dbg_ref, (size_t)t->W * t->H, out, count);
+    if (level < 0 || level >= t->L) return set_error(RT_ERR_ARG, "bad level");
+    return copy_image(t, t->d.dbg_icp_ref[level], (size_t)(t->W >> level) * (t->H >> level), out, count);
 }
 
 extern "C" int rt_set_filtered_reference(rt_tracker* t, int flag) {

@@ -93,6 +93,23 @@ class PhotoTracker:
         self.L.rt_synchronize(self.h)
         return bool(r)
 
+    def update_error_image(self):
+        check(self.L.rt_update_error_image(self.h), "rt_update_error_image")
+
+    def _image(self, fn, level):
+        w, h = (self.W, self.H) if self.get_mode() == PHOTOMETRIC else (self.W >> level, self.H >> level)
+        out = np.empty((h, w), np.float32)
+        check(fn(self.h, level, out.ctypes.data, out.size), fn.__name__)
+        return out
+
+    def get_debug_image(self, level=0):
+        """getDebugImage1C: the error image (host array)."""
+        return self._image(self.L.rt_get_debug_image, level)
+
+    def get_ref_image(self, level=0):
+        """getRefImage1C: the keyframe image for display (host array)."""
+        return self._image(self.L.rt_get_ref_image, level)
+
     def set_filtered_reference(self, flag):
         check(self.L.rt_set_filtered_reference(self.h, int(flag)), "rt_set_filtered_reference")
 

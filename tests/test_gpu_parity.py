@@ -208,3 +208,36 @@ def test_track_matches_oracle(mode, solver):
     # host pose bookkeeping (applyIncrement): T is the inverse of Trel
     assert np.allclose(tr.get_relative_pose().astype(np.float64) @ Tdev, np.eye(4), atol=1e-4)
     assert np.allclose(tr.get_global_pose(), np.linalg.inv(Tdev), atol=1e-3)
+
+
+@pytest.mark.parametrize("W,H,L,N,seed", CASES[:2])
+def test_inspection_images_bitexact(W, H, L, N, seed):
+    """updateErrorImage / getDebugImage1C / getRefImage1C against
+    oracle/debug_images.py on the oracle's own intermediate results, at a pose
+    part-way to the truth: photometric error at the selected pixels, the
+    keyframe's normalised depth, the geometric error of every level, and the
+    geometric reference images the reference never writes."""
+    from oracle import debug_images as di
+    print(f"case inspection images {W}x{H} seed {seed}", flush=True)
+    tr, st, Tgt = _setup(W, H, L, N, seed, mode=PHOTOMETRIC)
+    tr.debug_prepare()
+    st.features()
+    T = _trel_guess(Tgt)
+    tr.debug_set_trel(T)
+    tr.update_error_image()
+    res, _ = st.photo_residual(0, T)
+    exp = di.photo_error_image(st.sel, res, W * H)
+    assert np.array_equal(tr.get_debug_image().ravel(), exp)
+    assert np.array_equal(tr.get_ref_image().ravel(), di.ref_depth_image(st.ref_z, st.calib))
+    tr, st, Tgt = _setup(W, H, L, N, seed, mode=GEOMETRIC)
+    tr.debug_prepare()
+    st.clouds()
+    tr.debug_set_trel(T)
+    tr.update_error_image()
+    for l in range(L):
+        w, h = W >> l, H >> l
+        e = di.icp_error_image(st.rpts[l], st.rmask[l], st.cpts[l], st.cmask[l], w, h, T, st.calib, l)
+        got = tr.get_debug_image(l).ravel()
+        assert np.array_equal(got, e), f"geometric error image level {l}"
+        assert (got > 0).mean() > 0.2
+        assert np.all(tr.get_ref_image(l) == 0)

@@ -561,3 +561,54 @@ def test_cloud_downsampling_masked_mean_on_a_plane():
     inner = M.copy(); inner[0, :] = inner[-1, :] = False; inner[:, 0] = inner[:, -1] = False
     assert np.allclose(D[inner][:, :3] @ n, d, atol=1e-6)
     assert np.allclose(np.abs(D[inner][:, 3:] @ n), 1.0, atol=1e-9)
+
+
+def test_debug_images_closed_forms():
+    """The inspection images' restatement (oracle/debug_images.py): the
+    normalised keyframe depth, the photometric error at the selected pixels,
+    and the geometric error of a plane moved along the optical axis."""
+    from oracle import debug_images as di
+    cal = np.zeros(96, np.float32)
+    cal[62], cal[63] = 300.0, 7000.0
+    z = np.array([0.0, -300.0, -301.0, -3650.0, -6999.0, -7000.0, -9000.0, 5.0], np.float32)
+    got = di.ref_depth_image(z, cal)
+    exp = np.array([0, 0, 1 / 6700, 3350 / 6700, 6699 / 6700, 0, 0, 0])
+    np.testing.assert_allclose(got, exp, atol=1e-6)
+    img = di.photo_error_image([3, 0], np.array([0.01, -0.5], np.float32), 5)
+    assert list(img) == [1.0, 0.0, 0.0, np.float32(0.01) * np.float32(25.0), 0.0]
+    # a fronto-parallel plane at z = -1000 seen by a pinhole KL; the pose moves
+    # it 15 mm toward the camera: every selected point differs by 15 mm -> 0.5
+    w, h = 40, 30
+    f, cx, cy = 30.0, 19.5, 14.5
+    cal[43:52] = [-f, 0, cx, 0, f, cy, 0, 0, 1]
+    ys, xs = np.mgrid[0:h, 0:w].astype(np.float32)
+    zz = np.full((h, w), -1000.0, np.float32)
+    pts = np.zeros((h, w, 6), np.float32)
+    pts[..., 0] = (xs - cx) * zz / -f
+    pts[..., 1] = (ys - cy) * zz / f
+    pts[..., 2] = zz
+    mask = np.ones(w * h, np.int32)
+    mask[5] = 0
+    T = np.eye(4, dtype=np.float32)
+    T[2, 3] = 15.0
+    e = di.icp_error_image(pts, mask, pts, np.ones(w * h, np.int32), w, h, T, cal, 0)
+    assert e[5] == 0.0
+    inner = e.reshape(h, w)[2:-2, 2:-2]
+    np.testing.assert_allclose(inner, 0.5, atol=1e-5)
+    T[2, 3] = 60.0
+    e = di.icp_error_image(pts, mask, pts, np.ones(w * h, np.int32), w, h, T, cal, 0)
+    assert np.all(e.reshape(h, w)[2:-2, 2:-2] == 1.0)
+    e = di.icp_error_image(pts, mask, pts, np.zeros(w * h, np.int32), w, h, T, cal, 0)
+    assert np.all(e == 0.0)
+    # a surface whose depth changes along x: identity pose maps each point onto
+    # its own pixel (a wrong reprojection index would see 5 mm per column)
+    zt = (-1000.0 - 5.0 * xs).astype(np.float32)
+    pts[..., 0] = (xs - cx) * zt / -f
+    pts[..., 1] = (ys - cy) * zt / f
+    pts[..., 2] = zt
+    e = di.icp_error_image(pts, np.ones(w * h, np.int32), pts, np.ones(w * h, np.int32), w, h, np.eye(4), cal, 0)
+    assert np.all(e.reshape(h, w)[1:-1, 1:-1] == 0.0)
+    shifted = pts.copy()
+    shifted[..., :3] = np.roll(pts[..., :3], 1, axis=1)
+    e = di.icp_error_image(pts, np.ones(w * h, np.int32), shifted, np.ones(w * h, np.int32), w, h, np.eye(4), cal, 0)
+    np.testing.assert_allclose(e.reshape(h, w)[1:-1, 2:-1], 5.0 / 30.0, rtol=1e-5)
