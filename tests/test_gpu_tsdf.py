@@ -86,6 +86,12 @@ def test_resample_matches_oracle_bitexact():
         dst.resample_from(src, T)
         odst.resample_from(osrc, T)
         assert _bits_equal(dst.read(), odst.data)
+    # the ray cast of a resampled grid (its occupancy is rebuilt by the resampling)
+    for refine in (True, False):
+        got = dst.raycast(UNFUSED, K, refine)
+        exp = odst.raycast(UNFUSED, K, refine)
+        for name, a, b in zip(("xyz", "normals", "shade"), got, exp):
+            assert _bits_equal(a, b), f"{name} of the resampled grid, refine={refine}"
     # the resampled grid, ray cast from the matching pose, sees the same scene
     T = POSES[1]
     dst.resample_from(src, T)
@@ -105,6 +111,8 @@ def test_reset_and_argument_errors():
     g.reset()
     r = g.read()
     assert np.all(r[..., 0] == 1) and np.all(r[..., 1] == 0)
+    xyz, _, _ = g.raycast(np.eye(4, dtype=np.float32), K, True)
+    assert np.all(xyz == 0)
     other = VoxelGrid(default_params(resolution=64, width=W, height=H))
     with pytest.raises(RuntimeError):
         other.resample_from(g, np.eye(4))
