@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 #include <algorithm>
 #include <string>
@@ -60,6 +61,10 @@ struct rt_tracker {
     int refFrameNumber = 0, currentFrame = 0, referenceUpdates = 0;
     float T[16];
     double Tinc[16], global[16];
+    // the Gauss-Newton launches of one frame, captured once per configuration
+    // (mode, levels, iterations, solver settings) and replayed
+    hipGraphExec_t est_graph = nullptr;
+    int est_key[10] = {-1};
 };
 
 static void ident(float* M) { for (int i = 0; i < 16; i++) M[i] = (i % 5 == 0) ? 1.0f : 0.0f; }
@@ -117,6 +122,8 @@ static int dalloc(rt_tracker* t, T** p, size_t count) {
 
 static void free_all(rt_tracker* t) {
     if (t->stream) hipStreamSynchronize(t->stream);
+    if (t->est_graph) hipGraphExecDestroy(t->est_graph);
+    t->est_graph = nullptr;
     for (void* q : t->allocs) hipFree(q);
     t->allocs.clear();
     if (t->trel_host) hipHostFree(t->trel_host);
@@ -461,8 +468,7 @@ static int preprocess(rt_tracker* t, bool seq) {
 }
 
 // gpuEstimateIncrement (phototracker.cpp:666-689)
-static int estimate(rt_tracker* t) {
-    SolveCfg c = solve_cfg(t);
+static void estimate_launches(rt_tracker* t, const SolveCfg& c) {
     int m = t->mode;
     for (int layer = t->L - 1; layer >= 0; layer--) {
         int n = t->prm.iters[t->L - 1 - layer];
@@ -478,6 +484,34 @@ static int estimate(rt_tracker* t) {
                 launch_icp_system(t->d, layer, c, true, t->stream);
             }
         }
+    }
+}
+
+// gpuEstimateIncrement (phototracker.cpp:666-689): every level's iterations.
+// The launches take only fixed device buffers and the solve settings, so they
+// are captured into a graph once per configuration and replayed per frame
+// (RT_EAGER set in the environment launches them one by one instead).
+static int estimate(rt_tracker* t) {
+    SolveCfg c = solve_cfg(t);
+    int lam_bits;
+    memcpy(&lam_bits, &c.lambda, sizeof(lam_bits));
+    const int key[10] = {t->mode, t->L, t->prm.iters[0], t->prm.iters[1], t->prm.iters[2], t->prm.iters[3],
+                         c.solver, c.robust, c.mad_bins, lam_bits};
+    if (getenv("RT_EAGER")) {
+        estimate_launches(t, c);
+    } else {
+        if (!t->est_graph || memcmp(key, t->est_key, sizeof(key))) {
+            if (t->est_graph) { HIP_TRY(hipGraphExecDestroy(t->est_graph)); t->est_graph = nullptr; }
+            hipGraph_t g;
+            HIP_TRY(hipStreamBeginCapture(t->stream, hipStreamCaptureModeThreadLocal));
+            estimate_launches(t, c);
+            HIP_TRY(hipStreamEndCapture(t->stream, &g));
+            hipError_t e = hipGraphInstantiate(&t->est_graph, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) { t->est_graph = nullptr; return set_error(RT_ERR_HIP, "graph instantiate"); }
+            memcpy(t->est_key, key, sizeof(key));
+        }
+        HIP_TRY(hipGraphLaunch(t->est_graph, t->stream));
     }
     HIP_TRY(hipGetLastError());
     t->currentFrame++;
