@@ -13,6 +13,15 @@
 #include "rt_host.hpp"
 #include "rt_tsdf.h"
 
+// ray-cast workgroup: RAY_TX x RAY_TY pixels (one ray each). One wave per
+// workgroup spreads a 320x240 image's 1200 waves evenly over the CUs (a sweep
+// of 16x16, 8x8, 16x4, 8x4 and 32x8: 145, 139, 133, 147, 143 us per frame)
+#ifndef RAY_TX
+#define RAY_TX 16
+#endif
+#ifndef RAY_TY
+#define RAY_TY 4
+#endif
 #ifndef TSDF_AHEAD
 #define TSDF_AHEAD 8  // voxels of a ray whose loads are in flight together
 #endif
@@ -159,9 +168,9 @@ __device__ __forceinline__ float linear_zero(float tA, float dA, float tB, float
 
 // rayCastVoxelsKernel with initRay, intersectVoxelGridCuda and
 // traverseVoxelsCuda (VoxelGrid_cuda.cu:118-164, 633-770)
-__global__ __launch_bounds__(256) void k_tsdf_raycast(const float2* __restrict__ g, TsdfArgs a, int refine,
-                                                      float4* __restrict__ out) {
-    const int xi = blockIdx.x * 16 + (threadIdx.x & 15), yi = blockIdx.y * 16 + (threadIdx.x >> 4);
+__global__ __launch_bounds__(RAY_TX * RAY_TY) void k_tsdf_raycast(const float2* __restrict__ g, TsdfArgs a, int refine,
+                                                                  float4* __restrict__ out) {
+    const int xi = blockIdx.x * RAY_TX + (threadIdx.x % RAY_TX), yi = blockIdx.y * RAY_TY + (threadIdx.x / RAY_TX);
     if (xi >= a.w || yi >= a.h) return;
     const float* K = a.K;
     const float* T = a.T;
@@ -437,7 +446,8 @@ extern "C" int rt_tsdf_raycast(rt_tsdf* g, const float* T, const float* K, int r
     if (hipSetDevice(g->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
     const TsdfArgs a = tsdf_args(g, T, K);  // the ray cast walks from the camera: camera -> world
     dim3 grid((g->p.width + 15) / 16, (g->p.height + 15) / 16);
-    k_tsdf_raycast<<<grid, 256, 0, g->stream>>>(g->grid, a, refine ? 1 : 0, (float4*)xyz);
+    dim3 rgrid((g->p.width + RAY_TX - 1) / RAY_TX, (g->p.height + RAY_TY - 1) / RAY_TY);
+    k_tsdf_raycast<<<rgrid, RAY_TX * RAY_TY, 0, g->stream>>>(g->grid, a, refine ? 1 : 0, (float4*)xyz);
     k_tsdf_shade<<<grid, 256, 0, g->stream>>>((const float4*)xyz, g->p.width, g->p.height, (float4*)shade,
                                                (float4*)normals);
     if (hipGetLastError() != hipSuccess) return set_error(RT_ERR_HIP, "rt_tsdf_raycast");
