@@ -14,7 +14,7 @@ Workload "small": the same without the network, motions <= 3 deg / 50 mm,
 identity start.
 
 Plain = the same kernels one pair at a time from identity. A pair has
-converged when its pose is within 0.1 deg / 1 mm of the judge: the pose Gauss-
+converged when its pose is within 0.1 deg / 1 mm of the anchor: the pose Gauss-
 Newton converges to from the true motion. Prints one JSON line.
 """
 import argparse
@@ -183,7 +183,7 @@ def main():
         return
 
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: runs on one GPU; --gpus {args.gpus} is not supported "
               "(pairs are independent: shard them across GPUs and gather poses)", file=sys.stderr)
         sys.exit(2)
 
@@ -233,10 +233,10 @@ def main():
         log(f"track-only: {args.steps} tracking launches over {B} pairs done")
         return
 
-    # ---- judge: Gauss-Newton from the true motion (untimed)
-    judge, _, _, judge_st = bt.track(prepared, init=list(gt))
-    rj, tj = pose_errors(judge, gt)
-    log(f"judge (GN from truth): median {np.median(rj):.4f} deg {np.median(tj):.3f} mm vs truth")
+    # ---- anchor: Gauss-Newton from the true motion (untimed)
+    anchor, _, _, _ = bt.track(prepared, init=list(gt))
+    rj, tj = pose_errors(anchor, gt)
+    log(f"anchor (GN from truth): median {np.median(rj):.4f} deg {np.median(tj):.3f} mm vs truth")
 
     # ---- plain single-pair tracker (same kernels, one pair at a time, identity start)
     P = min(args.plain_pairs, B)
@@ -304,9 +304,9 @@ def main():
         r, t = pose_errors(T, ref)
         ok = (r <= 0.1) & (t <= 1.0)
         return ok, r, t
-    ok_b, rb, tb = conv(Trel_b, judge)
-    ok_i, _, _ = conv(ident, judge)
-    ok_p, rp, tp = conv(np.array(plain_T), judge[:P])
+    ok_b, rb, tb = conv(Trel_b, anchor)
+    ok_i, _, _ = conv(ident, anchor)
+    ok_p, rp, tp = conv(np.array(plain_T), anchor[:P])
     # the batched and the plain tracker sum the normal equations in different
     # orders, so they agree to rounding where both converge; a diverging pair
     # amplifies the difference, so the comparison is over the pairs both converge on
@@ -315,7 +315,7 @@ def main():
     rtb, ttb = pose_errors(Trel_b, gt)
     log(f"{'learned+batched' if learned else 'batched'}: {B} pairs, {step_s * 1e3:.2f} ms/step -> {value:.0f} frames/s "
         f"({value / plain_fps:.1f}x plain); kb_track {k_ms:.2f} ms, {achieved:.0f} GB/s algorithmic")
-    log(f"converged (<=0.1 deg / 1 mm of the judge): {'learned+batched' if learned else 'batched'} {ok_b.mean() * 100:.1f}%, "
+    log(f"converged (<=0.1 deg / 1 mm of the anchor): {'learned+batched' if learned else 'batched'} {ok_b.mean() * 100:.1f}%, "
         f"batched from identity {ok_i.mean() * 100:.1f}%, plain from identity {ok_p.mean() * 100:.1f}% (of {P}); "
         f"batched vs plain from identity, {int(both.sum())} pairs both converge on: max {ra.max():.2e} deg {ta.max():.2e} mm")
 
@@ -362,7 +362,7 @@ def main():
                         "plain_from_identity": round(float(ok_p.mean()), 4), "plain_sample": P},
         "pose_error": {"converged_median_deg_vs_truth": float(np.median(rtb[ok_b])) if ok_b.any() else None,
                        "converged_median_mm_vs_truth": float(np.median(ttb[ok_b])) if ok_b.any() else None,
-                       "judge_median_deg_vs_truth": float(np.median(rj)), "judge_median_mm_vs_truth": float(np.median(tj)),
+                       "anchor_median_deg_vs_truth": float(np.median(rj)), "anchor_median_mm_vs_truth": float(np.median(tj)),
                        "batched_vs_plain_identity_pairs": int(both.sum()),
                        "batched_vs_plain_identity_max_deg": float(ra.max()),
                        "batched_vs_plain_identity_max_mm": float(ta.max()), "vs_fp64_oracle": fp64},

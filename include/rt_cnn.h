@@ -3,7 +3,7 @@
  * Gauss-Newton tracker at large motions.
  *
  * No counterpart in the reference (it tracks from the previous pose only,
- * phototracker.cpp:704-718); the plain tracker stays the fallback and the judge.
+ * phototracker.cpp:704-718); the plain tracker stays the fallback and the arbiter.
  * Convolutions run on the matrix cores (bfloat16 operands, FP32 accumulation);
  * the network is trained on the GPU by the forward and backward kernels behind
  * rt_cnn_train_step.

@@ -4,7 +4,7 @@ relocalisation, composed from the library's C ABI.
 Per frame: the pose network proposes the current camera relative to the
 keyframe (from the raw keyframe and current frames) and the single-pair
 tracker (bi-objective by default) runs Gauss-Newton from that proposal. The
-plain method stays fallback and judge: when the frame's weighted residual RMS
+plain method stays the fallback and the arbiter: when the frame's weighted residual RMS
 exceeds mean + reloc_sigma * std of the recent frames, the frame is also
 tracked from the plain start (the previous estimate) and the lower RMS wins.
 If the RMS still spikes, the frame is tracked against up to reloc_keyframes

@@ -2,7 +2,7 @@
 reference's sample data (bin/data/sequence/bayer_rgbimageNNNN.ppm,
 rawdepthNNNN.ppm and bin/data/calib.xml): the raw 8-bit Bayer mosaics and
 11-bit Kinect disparity frames 1-4, unchanged, and the calibration file. Runs
-in the build container only (the reference tree is not on the GPU box); the
+in the build container only (the reference tree is not present where the GPU tests run); the
 fixture holds data only.
 """
 import os

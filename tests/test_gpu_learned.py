@@ -3,7 +3,7 @@
 started from the network's proposals converges far more often than the same
 tracker started from identity. A pair has converged when it ends within
 0.1 deg / 1 mm of the pose Gauss-Newton reaches from the true motion (the
-judge). Uses the committed weights recon_amd/data/pose_cnn.npz."""
+anchor). Uses the committed weights recon_amd/data/pose_cnn.npz."""
 import numpy as np
 import pytest
 
@@ -25,14 +25,14 @@ def test_learned_proposals_raise_the_convergence_rate():
                   cur_depth=d.ptr + (2 * i + 1) * npx * 2, width=W, height=H, calib=cal) for i in range(B)]
     bt = BatchTracker(B, W, H, mode=PHOTOMETRIC, n_select=8192, iters=(1, 3, 10, 5))
     prep = bt.prepare(pairs)
-    judge, _, _, _ = bt.track(prep, init=list(gt))
+    anchor, _, _, _ = bt.track(prep, init=list(gt))
     ident, _, _, _ = bt.retrack(np.tile(np.eye(4).ravel(), B), B)
     net = PoseNet.load(max_batch=B)
     bt.propose(net)
     learned, _, _, st = bt.retrack(None, B)
 
     def rate(T):
-        r, t = bench.pose_errors(T, judge)
+        r, t = bench.pose_errors(T, anchor)
         return float(np.mean((r <= 0.1) & (t <= 1.0)))
 
     ri, rl = rate(ident), rate(learned)
