@@ -114,7 +114,7 @@ class SequenceTracker:
         return [kfs[i] for i in chosen]
 
     def _relocalise(self, g, d, k, keyframes, G_est):
-        """Track frame k against the recent keyframes in one batched launch, then
+        """Track frame k against the candidate keyframes in one batched launch, then
         re-track the best reloc_verify of them with the second single-pair
         tracker; returns (keyframe index, its global pose, pose relative to it,
         rms) of the best one that agrees with the estimate G_est, or None."""

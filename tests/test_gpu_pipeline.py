@@ -2,8 +2,9 @@
 (<= 1 deg, <= 7.5 mm per frame) and fast (<= 8 deg, <= 60 mm per frame)
 segments of 100 frames: keyframe fusion on, learned proposals warm-starting
 every frame, the plain start as fallback when the residual RMS exceeds mean +
-3 sigma of the recent frames, then batched relocalisation against the 64 most
-recent keyframes, bi-objective tracking (lambda 0.5, 61440 points). Compared with the plain tracker (same
+3 sigma of the recent frames, then batched relocalisation against up to 64
+keyframes (those the keyframe search finds near the estimate, then the most
+recent ones), bi-objective tracking (lambda 0.5, 61440 points). Compared with the plain tracker (same
 keyframe policy, starting from the previous estimate, no relocalisation):
 frames/s, the fraction of frames whose motion is recovered within 0.5 deg /
 5 mm, and the drift (RMS of the 100-frame relative pose error) against ground
