@@ -424,18 +424,28 @@ struct TrackCfg {
     int robust;
     int mad_bins;
     float scale_out;
+    int pc_cap;  // point-cache capacity (points; PC variants)
+    int rz_len;  // length of the RZ residual array (the batch's largest point count)
 };
 
 
 // RZ: residual and zWeight of each point kept in registers (0) or in LDS (1,
 // at most RZ_MAX points per pair), trading LDS for occupancy.
 constexpr int RZ_MAX = 8192;
+// PC: the first cfg.pc_cap points of a pair keep {x, y, z, reference gray of
+// the level} in LDS across iterations (after the RZ residuals, when both are
+// on). The record is filled by each level's first residual pass (xyz by the
+// pair's first pass), so later iterations read 16 B of LDS instead of 20 B of
+// point stream from HBM; points beyond the capacity stream as before. Each
+// thread reads and writes only the entries of its own points, so the cache
+// needs no barrier of its own.
+constexpr int PC_MAX = 8192;
 // WPE: waves per SIMD the register allocation must allow (1: no constraint
 // beyond the workgroup size). Two 512-thread workgroups per CU (WPE 4), which
 // would overlap one pair's serial phases with the other's point passes, were
 // measured slower than one 1024-thread workgroup at 640x480: the kernel is
 // bound by memory traffic, not by those phases.
-template <int BS, int PPT, int RZ, int WPE = 1>
+template <int BS, int PPT, int RZ, int WPE = 1, int PC = 0>
 __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__ P, const int* __restrict__ order,
                                                     int npairs, int* queue, TrackCfg cfg) {
     constexpr int NW = BS / 64;
@@ -450,7 +460,9 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
     __shared__ float lut[256];
     // RZ: dynamic LDS after every static array, so that the static arrays keep
     // small addresses (immediate offsets from one base register)
-    extern __shared__ float2 rz_lds[];
+    extern __shared__ float4 dyn_lds[];
+    float2* rz_lds = reinterpret_cast<float2*>(dyn_lds);
+    float4* pcache = dyn_lds + (RZ ? cfg.rz_len / 2 : 0);
     const int t = threadIdx.x, wave = t >> 6, lane = lane_id();
     load_lut(lut);
     for (;;) {
@@ -464,6 +476,8 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
         if (p < 0) break;
         const PairDesc& d = P[p];
         const int W = d.W, H = d.H, L = d.L, N = d.N;
+        const int C = PC ? min(N, cfg.pc_cap) : 0;  // points held in the cache
+        bool xyz_cached = false;                 // set after the pair's first residual pass
         if (t < 96) cal[t] = d.calib[t];
         if (t < 16) T[t] = d.trel_in[t];
         if (t == 0) s_ok = 1;
@@ -489,8 +503,23 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
                 // residual pass: r and zWeight stay in registers (RZ 0) or go to LDS (RZ 1)
                 float r[RZ ? 1 : PPT], zw[RZ ? 1 : PPT];
                 auto residual = [&](int i, float& rr, float& zz) {
-                    const float4 p4 = ldg(d.pts, i);
-                    const float ir = ldg(jv.iref, i);
+                    float4 p4;
+                    float ir;
+                    if (PC && i < C) {
+                        if (it == 0) {
+                            // the level's first pass: xyz from HBM on the pair's first pass,
+                            // else from the cache; the level's reference gray from HBM
+                            p4 = xyz_cached ? pcache[i] : ldg(d.pts, i);
+                            ir = ldg(jv.iref, i);
+                            pcache[i] = make_float4(p4.x, p4.y, p4.z, ir);
+                        } else {
+                            p4 = pcache[i];
+                            ir = p4.w;
+                        }
+                    } else {
+                        p4 = ldg(d.pts, i);
+                        ir = ldg(jv.iref, i);
+                    }
                     const U16DepthSrc zs{d.d16[1]};
                     if (layer == 0) {
                         const U8Src g{d.g8[1], lut};
@@ -519,6 +548,7 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
                         if (i < N) residual(i, r[k], zw[k]);
                     }
                 }
+                xyz_cached = true;
                 __syncthreads();
                 // median bin: first bin whose running count exceeds N/2 (histogram64.cu:48-61)
                 if (wave == 0) {
@@ -612,7 +642,7 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
 }
 
 typedef void (*TrackFn)(const PairDesc*, const int*, int, int*, TrackCfg);
-struct Variant { int bs, ppt, rz, wpe; TrackFn fn; };
+struct Variant { int bs, ppt, rz, wpe; TrackFn fn; int pc; };
 static const Variant kVariants[] = {
     {256, 8, 0, 1, kb_track<256, 8, 0>},     {256, 16, 0, 1, kb_track<256, 16, 0>},
     {256, 32, 0, 1, kb_track<256, 32, 0>},   {512, 4, 0, 1, kb_track<512, 4, 0>},
@@ -624,6 +654,13 @@ static const Variant kVariants[] = {
     {256, 2, 1, 1, kb_track<256, 2, 1>},     {256, 4, 1, 1, kb_track<256, 4, 1>},
     {512, 2, 1, 1, kb_track<512, 2, 1>},     {512, 4, 1, 1, kb_track<512, 4, 1>},
     {1024, 2, 1, 1, kb_track<1024, 2, 1>},
+    // point-cache variants; pc = capacity in points (16 B of LDS each). With LDS
+    // residuals, 5632 points fill the 160 KB of LDS: 8 KB static + 64 KB of
+    // residuals for 8192 points + 88 KB of cache.
+    {1024, 8, 0, 1, kb_track<1024, 8, 0, 1, 1>, 8192},  {1024, 16, 0, 1, kb_track<1024, 16, 0, 1, 1>, 8192},
+    {512, 16, 0, 1, kb_track<512, 16, 0, 1, 1>, 8192},
+    {1024, 2, 1, 1, kb_track<1024, 2, 1, 1, 1>, 5632},   {1024, 4, 1, 1, kb_track<1024, 4, 1, 1, 1>, 5632},
+    {512, 4, 1, 1, kb_track<512, 4, 1, 1, 1>, 5632},     {1024, 2, 1, 1, kb_track<1024, 2, 1, 1, 1>, 5120},
 };
 
 }  // namespace rt
@@ -734,8 +771,10 @@ static void launch_prep(rt_batch* b, hipStream_t s) {
     kb_ref_points<<<dim3((b->maxN + 255) / 256, n), 256, 0, s>>>(b->desc_dev);
 }
 
-static TrackCfg track_cfg(const rt_batch* b) {
+static TrackCfg track_cfg(const rt_batch* b, const Variant& v) {
     TrackCfg c;
+    c.pc_cap = v.pc;
+    c.rz_len = b->maxN;
     c.solver = b->prm.solver;
     c.robust = b->prm.robust;
     c.mad_bins = b->prm.mad_bins;
@@ -743,25 +782,31 @@ static TrackCfg track_cfg(const rt_batch* b) {
     return c;
 }
 
-static size_t track_dyn_lds(const Variant& v, int maxN) { return v.rz ? sizeof(float2) * (size_t)maxN : 0; }
+static size_t track_dyn_lds(const Variant& v, int maxN) {
+    size_t n = 0;
+    if (v.rz) n += sizeof(float2) * (size_t)maxN;
+    if (v.pc) n += sizeof(float4) * (size_t)std::min(maxN, v.pc);
+    return n;
+}
 
-// RZ variants need more dynamic LDS than the default limit allows
-static bool rz_variant_ready(const Variant& v, int maxN) {
+// RZ and point-cache variants need more dynamic LDS than the default limit allows
+static bool lds_variant_ready(const Variant& v, int maxN) {
     static bool tried[64] = {}, ok[64] = {};
     int idx = (int)(&v - kVariants);
     if (!tried[idx]) {
         tried[idx] = true;
-        ok[idx] = hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                      (int)(sizeof(float2) * RZ_MAX)) == hipSuccess;
+        const size_t need = (v.rz ? sizeof(float2) * RZ_MAX : 0) + sizeof(float4) * v.pc;
+        ok[idx] = hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) ==
+                  hipSuccess;
         (void)hipGetLastError();
     }
-    return ok[idx] && maxN <= RZ_MAX;
+    return ok[idx] && (!v.rz || maxN <= RZ_MAX);
 }
 
 static void launch_track(rt_batch* b, int variant, int grid, hipStream_t s) {
     kb_reset_queue<<<1, 64, 0, s>>>(b->queue);
     const Variant& v = kVariants[variant];
-    v.fn<<<grid, v.bs, track_dyn_lds(v, b->maxN), s>>>(b->desc_dev, b->order_dev, b->n, b->queue, track_cfg(b));
+    v.fn<<<grid, v.bs, track_dyn_lds(v, b->maxN), s>>>(b->desc_dev, b->order_dev, b->n, b->queue, track_cfg(b, v));
 }
 
 static std::string tune_key(const rt_batch* b) {
@@ -781,6 +826,7 @@ static int autotune(rt_batch* b) {
     auto it = b->tuned.find(key);
     if (it != b->tuned.end()) { b->variant = it->second.first; b->grid = it->second.second; return RT_OK; }
     const char* force = getenv("RT_BATCH_VARIANT");
+    const int verbose = getenv("RT_VERBOSE") ? atoi(getenv("RT_VERBOSE")) : 0;
     hipEvent_t e0, e1;
     hipEventCreate(&e0); hipEventCreate(&e1);
     float best = 1e30f;
@@ -788,13 +834,15 @@ static int autotune(rt_batch* b) {
     for (int v = 0; v < (int)(sizeof(kVariants) / sizeof(kVariants[0])); v++) {
         const Variant& var = kVariants[v];
         if (var.rz) {
-            if (!rz_variant_ready(var, b->maxN)) continue;
+            if (!lds_variant_ready(var, b->maxN)) continue;
         } else {
             if (var.bs * var.ppt < b->maxN) continue;
             if (var.bs * var.ppt >= 4 * b->maxN && var.ppt > 4) continue;  // mostly idle lanes
+            if (var.pc && !lds_variant_ready(var, b->maxN)) continue;
         }
         if (force && atoi(force) != v) continue;
-        int per_cu_max = 2048 / var.bs;  // waves per CU stay within the hardware limit
+        // waves per CU stay within the hardware limit; a point-cache workgroup fills the CU's LDS
+        int per_cu_max = 2048 / var.bs;
         for (int k = 1; k <= 4 && k <= per_cu_max; k *= 2) {
             int grid = std::min(b->n, b->cus * k);
             launch_track(b, v, grid, b->stream);  // warm
@@ -804,6 +852,9 @@ static int autotune(rt_batch* b) {
             if (hipEventSynchronize(e1) != hipSuccess) { hipEventDestroy(e0); hipEventDestroy(e1); return set_error(RT_ERR_HIP, "autotune run failed"); }
             float ms = 0;
             hipEventElapsedTime(&ms, e0, e1);
+            if (verbose >= 2)
+                fprintf(stderr, "rt_batch autotune   bs %d ppt %d rz %d pc %d grid %d: %.3f ms\n", var.bs, var.ppt,
+                        var.rz, var.pc, grid, ms);
             if (ms < best) { best = ms; bv = v; bg = grid; }
             if (grid == b->n) break;
         }
@@ -813,8 +864,8 @@ static int autotune(rt_batch* b) {
     b->tuned[key] = {bv, bg};
     b->variant = bv; b->grid = bg;
     if (getenv("RT_VERBOSE"))
-        fprintf(stderr, "rt_batch autotune %s -> bs %d ppt %d rz %d wpe %d grid %d (%.3f ms)\n", key.c_str(),
-                kVariants[bv].bs, kVariants[bv].ppt, kVariants[bv].rz, kVariants[bv].wpe, bg, best);
+        fprintf(stderr, "rt_batch autotune %s -> bs %d ppt %d rz %d wpe %d pc %d grid %d (%.3f ms)\n", key.c_str(),
+                kVariants[bv].bs, kVariants[bv].ppt, kVariants[bv].rz, kVariants[bv].wpe, kVariants[bv].pc, bg, best);
     return RT_OK;
 }
 

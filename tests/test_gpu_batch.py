@@ -105,3 +105,40 @@ def test_batch_preparation_bitexact_against_oracle(N, rig):
             got = np.concatenate([jac[:4 * N].reshape(N, 4), jac[4 * N:6 * N].reshape(N, 2)], axis=1)
             assert np.array_equal(got, J), f"jacobian pair {i} level {l}"
             assert np.array_equal(jac[6 * N:], rec[:, color_slot[l]]), f"reference gray pair {i} level {l}"
+
+
+@pytest.mark.parametrize("N,iters", [(8192, (1, 3, 10, 5)), (12288, (0, 3, 4, 2))])
+def test_batch_kernel_variants_agree(N, iters, monkeypatch, capfd):
+    """Every variant of the persistent kernel the autotuner may pick, forced one
+    at a time: a thread sums its points in the same order (t, t + BS, ...)
+    whatever its unroll depth and wherever the residuals or the point records
+    are held (registers, LDS residuals, the LDS point cache), so variants of
+    one workgroup size agree bit for bit. N = 12288 runs past the point
+    cache's 8192 points, and a first level without iterations leaves the
+    cache to be filled by the next level."""
+    print(f"case batch variants N {N} iters {iters}", flush=True)
+    pairs, host, _ = _batch(6, theta=2.0, tmax=30.0, seed=60)
+    pairs = [p for p in pairs if p["width"] * p["height"] >= 2 * N]  # at most half the pixels are selected
+    monkeypatch.setenv("RT_VERBOSE", "1")
+    by_bs, kinds = {}, set()
+    for v in range(40):
+        monkeypatch.setenv("RT_BATCH_VARIANT", str(v))
+        bt = BatchTracker(len(pairs), 640, 480, mode=PHOTOMETRIC, n_select=N, iters=iters)
+        capfd.readouterr()
+        try:
+            Trel, _, rms, st = bt.track(pairs)
+        except RuntimeError:
+            continue  # no such variant, or it does not cover N points
+        finally:
+            bt.close()
+        line = [s for s in capfd.readouterr().err.splitlines() if "rt_batch autotune" in s][-1]
+        f = line.split("->")[1].split()
+        cfg = dict(zip(f[0:10:2], map(int, f[1:10:2])))
+        print(f"  variant {v}: {cfg}", flush=True)
+        assert np.all(st == 0)
+        kinds.add((cfg["rz"], cfg["pc"] > 0))
+        ref = by_bs.setdefault(cfg["bs"], (v, Trel, rms))
+        assert np.array_equal(Trel, ref[1]) and np.array_equal(rms, ref[2]), f"variant {v} vs {ref[0]}"
+    assert (0, True) in kinds and (0, False) in kinds
+    if N <= 8192:
+        assert (1, True) in kinds and (1, False) in kinds
