@@ -148,14 +148,15 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     uniform_calib(cal, cl);
     const U16DepthSrc zs{d.d16[0]};
     const U8Src gs{d.g8[0], lut};
+    const InvFocal inv = inv_focal(cl);
     RT_GLOBAL uint8_t* bins = gmem(d.bins);
     auto pixels = [&](auto simple) {
 #pragma unroll
         for (int k = 0; k < ROWS; k++) {
             const int y = y0 + k;
             if (x < W && y < H) {
-                PixFeat f = pixel_features_tiled<decltype(simple)::value>(-(float)zr[k], zs, gs, tile, W, H, x, y, cl);
-                unsigned b = grad_bin(f.gmag);
+                const float g = grad_mag_tiled<decltype(simple)::value>(-(float)zr[k], zs, gs, tile, W, H, x, y, cl, inv);
+                unsigned b = grad_bin(g);
                 bins[(size_t)x + (size_t)y * W] = (uint8_t)b;
                 atomicAdd(&h[b * HC + (lane & (HC - 1))], 1u);
             }

@@ -210,6 +210,23 @@ __device__ __forceinline__ PixFeat pixel_features(const float* __restrict__ z, c
     return pixel_features(F32Src{z}, F32Src{g0}, W, H, xi, yi, c);
 }
 
+// a / b rounded to nearest, given y = RN(1/b) (an IEEE division, done once per
+// divisor): q = RN(a y), then one fused correction with the exact remainder
+// r = a - b q, q' = RN(q + r y). By Markstein's theorem (P. Markstein, IBM J.
+// Res. Dev. 34(1), 1990; Muller et al., Handbook of Floating-Point
+// Arithmetic, division with an FMA) q' is the correctly rounded quotient
+// when no intermediate overflows or underflows, so it equals the IEEE
+// division bit for bit, except that a = -0 over a positive b gives +0 (the
+// division gives -0). Three instructions instead of about eleven.
+__device__ __forceinline__ float div_rn(float a, float b, float y) {
+    const float q = a * y;
+    const float r = __builtin_fmaf(-b, q, a);
+    return __builtin_fmaf(r, y, q);
+}
+// RN(1/fx), RN(1/fy) of the depth camera (KL) for div_rn
+struct InvFocal { float fx, fy; };
+__device__ __forceinline__ InvFocal inv_focal(const float* c) { return InvFocal{1.0f / c[KL], 1.0f / c[KL + 4]}; }
+
 // A block of gray values staged in LDS: image pixels [x0, x0+w) x [y0, y0+h),
 // row pitch w, holding exactly the values the image source returns.
 struct LdsTile {
@@ -219,19 +236,27 @@ struct LdsTile {
         return xa >= x0 && ya >= y0 && xb < x0 + w && yb < y0 + h;
     }
 };
-// pixel_features with the 4x4 tap neighbourhood read from an LDS tile when it
-// lies inside it (the same values, so the same results), else from the image;
-// zc is the pixel's own depth value z[xi + yi*W], loaded by the caller.
-// SIMPLE: the calibration satisfies calib_simple.
+// The gradient magnitude of pixel_features, with the 4x4 tap neighbourhood
+// read from an LDS tile when it lies inside it (the same values, so the same
+// results), else from the image; zc is the pixel's own depth value
+// z[xi + yi*W], loaded by the caller. The four divisions of the back-projection
+// and the projection run as div_rn (the same quotients; a zero coordinate may
+// change sign, which the projection maps to the same position: it is only
+// ever added to cx, cy or scaled into a sum). SIMPLE: the calibration
+// satisfies calib_simple.
 template <bool SIMPLE, class ZS, class GS>
-__device__ __forceinline__ PixFeat pixel_features_tiled(float zc, const ZS& z, const GS& g0, const LdsTile& tile,
-                                                        int W, int H, int xi, int yi, const float* c) {
-    PixFeat f;
-    f3 p3;
-    backproject(zc, xi, yi, c, p3);
+__device__ __forceinline__ float grad_mag_tiled(float zc, const ZS& z, const GS& g0, const LdsTile& tile, int W,
+                                                int H, int xi, int yi, const float* c, InvFocal inv) {
+    f3 p3{0.0f, 0.0f, 0.0f};
+    if (zc < -c[MIND] && zc > -c[MAXD]) {  // backproject
+        p3.x = div_rn(((float)xi - c[KL + 2]) * zc, c[KL], inv.fx);
+        p3.y = div_rn(((float)yi - c[KL + 5]) * zc, c[KL + 4], inv.fy);
+        p3.z = zc;
+    }
     if (p3.z < 0.0f) {
         f3 r3 = SIMPLE ? p3 : mul4(c + TLR, p3);
-        f2 pu{r3.x / r3.z, r3.y / r3.z};
+        const float iz = 1.0f / r3.z;
+        f2 pu{div_rn(r3.x, r3.z, iz), div_rn(r3.y, r3.z, iz)};
         f2 p = SIMPLE ? project_simple(pu, c + KR) : distort(pu, c + KCR, c + KR);
         if (in_bounds(p, W, H)) {
             int xdi = (int)p.x, ydi = (int)p.y;
@@ -245,25 +270,18 @@ __device__ __forceinline__ PixFeat pixel_features_tiled(float zc, const ZS& z, c
                 float a10 = q[-1], a11 = q[0], a12 = q[1], a13 = q[2];
                 float a20 = q[P - 1], a21 = q[P], a22 = q[P + 1], a23 = q[P + 2];
                 float a31 = q[2 * P], a32 = q[2 * P + 1];
-                float c1 = blend(a11, a12, a21, a22, fx, fy);
                 float cN = blend(a01, a02, a11, a12, fx, fy);
                 float cE = blend(a12, a13, a22, a23, fx, fy);
                 float cW = blend(a10, a11, a20, a21, fx, fy);
                 float cS = blend(a21, a22, a31, a32, fx, fy);
-                f.gx = (cE - cW) / 2.0f;
-                f.gy = (cS - cN) / 2.0f;
-                f.gmag = fminf(fabsf(f.gy) + fabsf(f.gx), 1.0f);
-                f.c1 = c1;
-                f.p = p3;
-                f.ok = true;
-                return f;
+                float gx = (cE - cW) / 2.0f;
+                float gy = (cS - cN) / 2.0f;
+                return fminf(fabsf(gy) + fabsf(gx), 1.0f);
             }
-            return pixel_features(z, g0, W, H, xi, yi, c);
+            return pixel_features(z, g0, W, H, xi, yi, c).gmag;
         }
     }
-    f.p = f3{0.0f, 0.0f, -c[MAXD]};
-    f.c1 = 0.0f; f.gx = 0.0f; f.gy = 0.0f; f.gmag = 0.0f; f.ok = false;
-    return f;
+    return 0.0f;
 }
 
 // gradient-magnitude bin (gpu_hist.cu:135, filterIndex256.cu:126), fused multiply-add
