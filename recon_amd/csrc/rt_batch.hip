@@ -195,7 +195,19 @@ __device__ __forceinline__ Bins8 load_bins8(const RT_GLOBAL uint8_t* bins, int b
 }
 // the lane's 8 bins at offset o of segments s0, s0 + SEL_WAVES, ...; the
 // uniform branch sits outside, so each path issues all its loads back to back
-constexpr int SEL_BS = 1024, SEL_WAVES = SEL_BS / 64, SEL_U = 4;
+// Workgroup size, segments in flight per wave and waves per SIMD of the
+// selection. 512 threads with two segments in flight fit in 64 VGPRs, so four
+// workgroups share a CU: measured 0.5 ms faster per 4096 pairs at 640x480
+// than one 1024-thread workgroup with four segments in flight (106 VGPRs).
+#ifndef RT_SEL_BS
+#define RT_SEL_BS 512
+#define RT_SEL_U 2
+#define RT_SEL_WPE 8
+#endif
+// SEL_BS >= 512: one thread per segment in the scans over the 512 segments
+constexpr int SEL_BS = RT_SEL_BS, SEL_WAVES = SEL_BS / 64, SEL_U = RT_SEL_U;
+static_assert(SEL_BS >= 512 && SEL_BS % 64 == 0, "kb_select needs a thread per segment");
+static_assert(512 % (SEL_WAVES * SEL_U) == 0, "the waves' segment groups tile the 512 segments");
 __device__ __forceinline__ void load_segments(Bins8* w, const RT_GLOBAL uint8_t* bins, int s0, int Pn, int o, bool vec) {
     if (vec) {
 #pragma unroll
@@ -280,7 +292,7 @@ __device__ __forceinline__ SelFlags sel_flags(Bins8 w, int nv, unsigned T, unsig
     f.ehi = bytes_eq(w.hi, T) & vhi;
     return f;
 }
-__global__ __launch_bounds__(SEL_BS) void kb_select(const PairDesc* __restrict__ P) {
+__global__ __launch_bounds__(SEL_BS, RT_SEL_WPE) void kb_select(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.x];
     __shared__ int cA[512], cE[512], sA[512], sE[512], pm[512];
     __shared__ int wtot[3][8];
