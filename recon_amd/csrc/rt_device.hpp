@@ -187,11 +187,15 @@ __device__ __forceinline__ PixFeat pixel_features(const ZS& z, const GS& g0, int
             float a10 = g0[o - 1], a11 = g0[o], a12 = g0[o + 1], a13 = g0[o + 2];
             float a20 = g0[o + W - 1], a21 = g0[o + W], a22 = g0[o + W + 1], a23 = g0[o + W + 2];
             float a31 = g0[o + 2 * W], a32 = g0[o + 2 * W + 1];
+            // the four shifted blends are formed before the rare re-reads, so
+            // that all twelve loads are issued together
             float c1 = blend(a11, a12, a21, a22, fx, fy);
-            float cN = yn == ydi - 1 ? blend(a01, a02, a11, a12, fx, fy) : bilerp(g0, W, xdi, yn, fx, fy);
-            float cE = xe == xdi + 1 ? blend(a12, a13, a22, a23, fx, fy) : bilerp(g0, W, xe, ydi, fx, fy);
-            float cW = xw == xdi - 1 ? blend(a10, a11, a20, a21, fx, fy) : bilerp(g0, W, xw, ydi, fx, fy);
-            float cS = ys == ydi + 1 ? blend(a21, a22, a31, a32, fx, fy) : bilerp(g0, W, xdi, ys, fx, fy);
+            float bN = blend(a01, a02, a11, a12, fx, fy), bE = blend(a12, a13, a22, a23, fx, fy);
+            float bW = blend(a10, a11, a20, a21, fx, fy), bS = blend(a21, a22, a31, a32, fx, fy);
+            float cN = yn == ydi - 1 ? bN : bilerp(g0, W, xdi, yn, fx, fy);
+            float cE = xe == xdi + 1 ? bE : bilerp(g0, W, xe, ydi, fx, fy);
+            float cW = xw == xdi - 1 ? bW : bilerp(g0, W, xw, ydi, fx, fy);
+            float cS = ys == ydi + 1 ? bS : bilerp(g0, W, xdi, ys, fx, fy);
             f.gx = (cE - cW) / 2.0f;
             f.gy = (cS - cN) / 2.0f;
             f.gmag = fminf(fabsf(f.gy) + fabsf(f.gx), 1.0f);

@@ -235,37 +235,54 @@ __device__ __forceinline__ void ref_point(int i, const ZS& z, const GS& g0,
         f3 r3 = mul4(c + TLR, f.p);
         f2 pu{r3.x / r3.z, r3.y / r3.z};
         f2 p1 = distort(pu, c + KCR, c + KR);
-        if (in_bounds(p1, W, H)) {
+        if (L > 1 && in_bounds(p1, W, H)) {
+            // Five bilinear samples (centre, N, W, E, S) per level with every tap
+            // clamped into the image, because the reference's taps can leave the
+            // image at the coarsest level near the border; together they read 12
+            // distinct taps of the clamped 4x4 neighbourhood: load each once. The
+            // taps of all levels are loaded before any is blended, so that their
+            // loads are in flight together (levels past L read level 1 and are
+            // discarded).
             f2 q = p1;
-            for (int l = 1; l < L; l++) {
+            float tp[3][12], fX[3], fY[3];
+#pragma unroll
+            for (int l = 1; l < 4; l++) {
                 q.x = 0.5f * q.x + -0.25f;
                 q.y = 0.5f * q.y + -0.25f;
-                int wl = W >> l, hl = H >> l;
+                const int lq = l < L ? l : 1;
+                int wl = W >> lq, hl = H >> lq;
                 int xdi = (int)q.x, ydi = (int)q.y;
-                float fX = q.x - (float)xdi, fY = q.y - (float)ydi;
-                // Five bilinear samples (centre, N, W, E, S) with every tap clamped
-                // into the image, because the reference's taps can leave the image
-                // at the coarsest level near the border; together they read 12
-                // distinct taps of the clamped 4x4 neighbourhood: load each once.
-                const RT_GLOBAL float* im = gmem(lv[l]);
+                fX[l - 1] = q.x - (float)xdi;
+                fY[l - 1] = q.y - (float)ydi;
+                const RT_GLOBAL float* im = gmem(lv[lq]);
                 int xs[4], ys[4];
 #pragma unroll
                 for (int a = 0; a < 4; a++) {
                     xs[a] = min(max(xdi - 1 + a, 0), wl - 1);
                     ys[a] = min(max(ydi - 1 + a, 0), hl - 1) * wl;
                 }
-                float a01 = im[xs[1] + ys[0]], a02 = im[xs[2] + ys[0]];
-                float a10 = im[xs[0] + ys[1]], a11 = im[xs[1] + ys[1]], a12 = im[xs[2] + ys[1]], a13 = im[xs[3] + ys[1]];
-                float a20 = im[xs[0] + ys[2]], a21 = im[xs[1] + ys[2]], a22 = im[xs[2] + ys[2]], a23 = im[xs[3] + ys[2]];
-                float a31 = im[xs[1] + ys[3]], a32 = im[xs[2] + ys[3]];
-                float cc = blend(a11, a12, a21, a22, fX, fY);  // at (xdi, ydi)
-                float cN = blend(a01, a02, a11, a12, fX, fY);  // (xdi, ydi - 1)
-                float cW = blend(a10, a11, a20, a21, fX, fY);  // (xdi - 1, ydi)
-                float cE = blend(a12, a13, a22, a23, fX, fY);  // (xdi + 1, ydi)
-                float cS = blend(a21, a22, a31, a32, fX, fY);  // (xdi, ydi + 1)
-                gx[l] = (cE - cW) / 2.0f;
-                gy[l] = (cS - cN) / 2.0f;
-                col[l] = cc;
+                float* t = tp[l - 1];
+                t[0] = im[xs[1] + ys[0]]; t[1] = im[xs[2] + ys[0]];
+                t[2] = im[xs[0] + ys[1]]; t[3] = im[xs[1] + ys[1]]; t[4] = im[xs[2] + ys[1]]; t[5] = im[xs[3] + ys[1]];
+                t[6] = im[xs[0] + ys[2]]; t[7] = im[xs[1] + ys[2]]; t[8] = im[xs[2] + ys[2]]; t[9] = im[xs[3] + ys[2]];
+                t[10] = im[xs[1] + ys[3]]; t[11] = im[xs[2] + ys[3]];
+            }
+            // every level is blended and kept by a select (levels past L stay 0),
+            // so that no level's loads sink into a branch of their own
+#pragma unroll
+            for (int l = 1; l < 4; l++) {
+                const float* t = tp[l - 1];
+                const float a01 = t[0], a02 = t[1], a10 = t[2], a11 = t[3], a12 = t[4], a13 = t[5];
+                const float a20 = t[6], a21 = t[7], a22 = t[8], a23 = t[9], a31 = t[10], a32 = t[11];
+                const float fx = fX[l - 1], fy = fY[l - 1];
+                float cc = blend(a11, a12, a21, a22, fx, fy);  // at (xdi, ydi)
+                float cN = blend(a01, a02, a11, a12, fx, fy);  // (xdi, ydi - 1)
+                float cW = blend(a10, a11, a20, a21, fx, fy);  // (xdi - 1, ydi)
+                float cE = blend(a12, a13, a22, a23, fx, fy);  // (xdi + 1, ydi)
+                float cS = blend(a21, a22, a31, a32, fx, fy);  // (xdi, ydi + 1)
+                gx[l] = l < L ? (cE - cW) / 2.0f : 0.0f;
+                gy[l] = l < L ? (cS - cN) / 2.0f : 0.0f;
+                col[l] = l < L ? cc : 0.0f;
             }
         }
     }
