@@ -13,9 +13,13 @@ loop of every pair in one persistent launch.
 Workload "small": the same without the network, motions <= 3 deg / 50 mm,
 identity start.
 
-Plain = the same kernels one pair at a time from identity. A pair has
-converged when its pose is within 0.1 deg / 1 mm of the anchor: the pose Gauss-
-Newton converges to from the true motion. Prints one JSON line.
+Plain = the single-pair tracker behind the PhotoTracker boundary (its own
+residual and system kernels, the same per-point arithmetic, a different
+summation order), one pair at a time from identity. A pair has converged when
+its pose is within 0.1 deg / 1 mm of the FP64 oracle tracker started at the
+true motion; convergence rates are measured on a sample of the pairs, and
+`value` is the whole batch's frames/s times the sampled convergence rate
+(converged frames/s). Prints one JSON line.
 """
 import argparse
 import ctypes
@@ -112,6 +116,25 @@ def copy_probe(mib):
     log(f"copy probe: {n} bytes copied")
 
 
+def host_cpu():
+    """The host's CPU model and the processing units this process may use
+    (`nproc`, which honours the affinity mask and OMP_NUM_THREADS)."""
+    import subprocess
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = int(subprocess.run(["nproc"], stdout=subprocess.PIPE, text=True, check=True).stdout)
+    except (OSError, ValueError, subprocess.CalledProcessError):
+        nproc = len(os.sched_getaffinity(0))
+    return model, nproc
+
+
 def cpu_baseline(g, d, npairs, W, H, L, N, iters, cal, seconds, threads):
     """The oracle's FP32 tracker over a sample of the same pairs, one pair per
     thread at a time, for about `seconds` of wall time; frames/s over all
@@ -122,7 +145,8 @@ def cpu_baseline(g, d, npairs, W, H, L, N, iters, cal, seconds, threads):
     npx = W * H
     gh = g.to_array(np.uint8, 2 * npairs * npx).reshape(2 * npairs, H, W)
     dh = d.to_array(np.uint16, 2 * npairs * npx).reshape(2 * npairs, H, W)
-    threads = max(1, min(threads, os.cpu_count() or 1))
+    model, nproc = host_cpu()
+    threads = max(1, min(threads, nproc)) if threads > 0 else nproc
     done = [0] * threads
     t0 = time.perf_counter()
     stop = t0 + seconds
@@ -146,8 +170,10 @@ def cpu_baseline(g, d, npairs, W, H, L, N, iters, cal, seconds, threads):
     el = time.perf_counter() - t0
     n = sum(done)
     return {"value": round(n / el, 2), "unit": "frames/s", "cores": threads, "kind": "port",
-            "sample": f"{n} pairs cycled from the first {npairs} {W}x{H} pairs of the batch, FP32 oracle tracker, "
-                      f"{threads} threads, {el:.1f} s wall"}
+            "cpu_model": model, "nproc": nproc, "logical_cpus": os.cpu_count(),
+            "sample": f"{n} pairs cycled from the first {npairs} {W}x{H} pairs of the batch, FP32 oracle tracker "
+                      f"(the reference's kernels restated on the CPU), one pair per thread, {threads} threads, "
+                      f"{el:.1f} s wall"}
 
 
 def main():
@@ -163,13 +189,14 @@ def main():
     ap.add_argument("--theta", type=float, default=None, help="max rotation (deg) of a pair")
     ap.add_argument("--tmax", type=float, default=None, help="max translation (mm) of a pair")
     ap.add_argument("--weights", default=os.path.join(ROOT, "recon_amd", "data", "pose_cnn.npz"))
-    ap.add_argument("--oracle-pairs", type=int, default=8, help="pairs checked against the FP64 oracle (0: skip)")
+    ap.add_argument("--oracle-pairs", type=int, default=256,
+                    help="pairs judged against the FP64 oracle tracker started at the true motion")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall time of the CPU baseline sample (0: skip)")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="CPU threads (0: nproc)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--extras", action="store_true",
-                    help="also run the single-pair, sequence and mixed-batch cases (off by default: this leg once "
-                         "faulted the GPU in its mixed-batch part, cause not yet found)")
+                    help="also run the single-pair and sequence cases (the mixed-size batch leg is disabled: it "
+                         "faulted the GPU once and is kept off the GPU until its cause is shown)")
     ap.add_argument("--track-only", action="store_true",
                     help="prepare the batch and run --steps tracking launches only (for counter collection)")
     ap.add_argument("--tsdf", action="store_true",
@@ -233,11 +260,6 @@ def main():
         log(f"track-only: {args.steps} tracking launches over {B} pairs done")
         return
 
-    # ---- anchor: Gauss-Newton from the true motion (untimed)
-    anchor, _, _, _ = bt.track(prepared, init=list(gt))
-    rj, tj = pose_errors(anchor, gt)
-    log(f"anchor (GN from truth): median {np.median(rj):.4f} deg {np.median(tj):.3f} mm vs truth")
-
     # ---- plain single-pair tracker (same kernels, one pair at a time, identity start)
     P = min(args.plain_pairs, B)
     tr = PhotoTracker(W, H, cal, mode=PHOTOMETRIC, n_select=N, iters=iters)
@@ -272,7 +294,7 @@ def main():
         Trel_b, _, rms, st = step()
     stream_sync(stream)
     step_s = (time.perf_counter() - t2) / args.steps
-    value = B / step_s
+    frames_per_s = B / step_s
 
     # ---- dominant kernel (the persistent tracking launch), HIP events on its stream
     ev0, ev1, ev2 = Event(), Event(), Event()
@@ -299,31 +321,44 @@ def main():
                "ms": round(c_ms, 3), "flops_per_pair": CNN_FLOPS_PER_PAIR,
                "note": "proposal launches (inputs + six convolutions + head + poses) timed together"}
 
-    # ---- convergence and errors
+    # ---- convergence and errors against the FP64 oracle tracker started at the
+    # true motion (the judge), on the first S pairs of the batch
+    S = max(P, min(args.oracle_pairs, B))
+    t3 = time.perf_counter()
+    ref64 = fp64_reference(g, d, gt, S, W, H, L, N, iters, cal, args.cpu_threads)
+    log(f"FP64 oracle from the true motion: {S} pairs in {time.perf_counter() - t3:.1f} s")
+
     def conv(T, ref):
         r, t = pose_errors(T, ref)
         ok = (r <= 0.1) & (t <= 1.0)
         return ok, r, t
-    ok_b, rb, tb = conv(Trel_b, anchor)
-    ok_i, _, _ = conv(ident, anchor)
-    ok_p, rp, tp = conv(np.array(plain_T), anchor[:P])
+    ok_b, rb, tb = conv(Trel_b[:S], ref64)
+    ok_i, _, _ = conv(ident[:S], ref64)
+    ok_p, rp, tp = conv(np.array(plain_T), ref64[:P])
+    rate = float(ok_b.mean())
+    value = frames_per_s * rate
     # the batched and the plain tracker sum the normal equations in different
     # orders, so they agree to rounding where both converge; a diverging pair
     # amplifies the difference, so the comparison is over the pairs both converge on
     both = ok_i[:P] & ok_p
     ra, ta = pose_errors(ident[:P][both], np.array(plain_T)[both]) if both.any() else (np.zeros(1), np.zeros(1))
-    rtb, ttb = pose_errors(Trel_b, gt)
-    log(f"{'learned+batched' if learned else 'batched'}: {B} pairs, {step_s * 1e3:.2f} ms/step -> {value:.0f} frames/s "
-        f"({value / plain_fps:.1f}x plain); kb_track {k_ms:.2f} ms, {achieved:.0f} GB/s algorithmic")
-    log(f"converged (<=0.1 deg / 1 mm of the anchor): {'learned+batched' if learned else 'batched'} {ok_b.mean() * 100:.1f}%, "
+    rtb, ttb = pose_errors(Trel_b[:S], gt[:S])
+    plain_conv_fps = plain_fps * float(ok_p.mean())
+    log(f"{'learned+batched' if learned else 'batched'}: {B} pairs, {step_s * 1e3:.2f} ms/step -> {frames_per_s:.0f} "
+        f"frames/s, {value:.0f} converged frames/s ({value / plain_fps:.1f}x plain); kb_track {k_ms:.2f} ms, "
+        f"{achieved:.0f} GB/s algorithmic")
+    log(f"converged (<=0.1 deg / 1 mm of the FP64 oracle from the truth, {S} pairs): "
+        f"{'learned+batched' if learned else 'batched'} {rate * 100:.1f}%, "
         f"batched from identity {ok_i.mean() * 100:.1f}%, plain from identity {ok_p.mean() * 100:.1f}% (of {P}); "
         f"batched vs plain from identity, {int(both.sum())} pairs both converge on: max {ra.max():.2e} deg {ta.max():.2e} mm")
-
-    # ---- FP64 oracle from the true motion on a sample: error of the converged results
-    fp64 = None
-    if args.oracle_pairs > 0:
-        fp64 = fp64_check(g, d, gt, Trel_b, ok_b, min(args.oracle_pairs, B), W, H, L, N, iters, cal, args.cpu_threads)
-        log(f"vs FP64 oracle: {fp64}")
+    fp64 = {"pairs": S, "converged": int(ok_b.sum()),
+            "converged_max_deg": float(rb[ok_b].max()) if ok_b.any() else None,
+            "converged_max_mm": float(tb[ok_b].max()) if ok_b.any() else None,
+            "converged_median_deg": float(np.median(rb[ok_b])) if ok_b.any() else None,
+            "converged_median_mm": float(np.median(tb[ok_b])) if ok_b.any() else None,
+            # error of every sampled result against the FP64 oracle (deg, mm), pair order
+            "per_pair_deg": [round(float(x), 5) for x in rb], "per_pair_mm": [round(float(x), 4) for x in tb]}
+    log(f"vs FP64 oracle (converged pairs): max {fp64['converged_max_deg']} deg {fp64['converged_max_mm']} mm")
 
     # ---- CPU baseline: the oracle's FP32 transcription on the host cores
     cpu = None
@@ -333,8 +368,7 @@ def main():
 
     extras = None
     if args.extras:
-        extras = {"single_pair_320x240": extra_single_pair(), "sequence_640x480": extra_sequence(),
-                  "mixed_batch": extra_mixed_batch(args.cpu_threads)}
+        extras = {"single_pair_320x240": extra_single_pair(), "sequence_640x480": extra_sequence()}
         for k, v in extras.items():
             log(f"{k}: {v}")
 
@@ -355,14 +389,16 @@ def main():
         "config": {"workload": f"{name}: {B} frame pairs at {W}x{H}, rotation <= {theta} deg, translation <= {tmax} mm, "
                                f"{N} points, {L} levels, iterations {list(iters[:L])}",
                    "pairs": B, "width": W, "height": H, "points": N, "levels": L, "theta_deg": theta, "tmax_mm": tmax},
+        "frames_per_s_all_pairs": round(frames_per_s, 1),
         "plain_fps": round(plain_fps, 1),
+        "plain_converged_fps": round(plain_conv_fps, 1),
+        # converged frames/s over the plain tracker's frames/s, whether or not its pairs converge
         "speedup_vs_plain": round(value / plain_fps, 2),
-        "convergence": {"criterion": "within 0.1 deg and 1 mm of Gauss-Newton started at the true motion",
-                        "this": round(float(ok_b.mean()), 4), "batched_from_identity": round(float(ok_i.mean()), 4),
+        "convergence": {"criterion": "within 0.1 deg and 1 mm of the FP64 oracle tracker started at the true motion",
+                        "sample": S, "this": round(rate, 4), "batched_from_identity": round(float(ok_i.mean()), 4),
                         "plain_from_identity": round(float(ok_p.mean()), 4), "plain_sample": P},
         "pose_error": {"converged_median_deg_vs_truth": float(np.median(rtb[ok_b])) if ok_b.any() else None,
                        "converged_median_mm_vs_truth": float(np.median(ttb[ok_b])) if ok_b.any() else None,
-                       "anchor_median_deg_vs_truth": float(np.median(rj)), "anchor_median_mm_vs_truth": float(np.median(tj)),
                        "batched_vs_plain_identity_pairs": int(both.sum()),
                        "batched_vs_plain_identity_max_deg": float(ra.max()),
                        "batched_vs_plain_identity_max_mm": float(ta.max()), "vs_fp64_oracle": fp64},
@@ -533,7 +569,11 @@ def extra_sequence():
 def extra_mixed_batch(threads):
     """256 pairs of mixed sizes (160x120, 320x240, 480x360, 640x480; motion <= 3
     deg / 50 mm) in one persistent launch, 8192 points each; frames/s of the
-    tracking step and per-pair error against the FP64 oracle on a sample."""
+    tracking step and per-pair error against the FP64 oracle on a sample.
+
+    Disabled: an earlier build faulted the GPU in this leg, and the case stays
+    off the GPU until the cause is shown and fixed."""
+    raise RuntimeError("the mixed-size batch leg is disabled: it faulted the GPU once and its cause is not yet shown")
     import threading
 
     import oracle
@@ -570,6 +610,10 @@ def extra_mixed_batch(threads):
         gts.append(Tgt)
         pairs.append(dict(ref_gray=bufs[0].ptr, ref_depth=bufs[1].ptr, cur_gray=bufs[2].ptr, cur_depth=bufs[3].ptr,
                           width=W, height=H, calib=synth.calib_block_np(W, H)))
+    # the frames were rendered on the null stream; the batch runs on a
+    # non-blocking stream that does not wait for it
+    from recon_amd.hip import hip
+    hip().hipDeviceSynchronize()
     bt = BatchTracker(n, 640, 480, mode=PHOTOMETRIC, n_select=8192, iters=(1, 3, 10, 5))
     prep = bt.prepare(pairs)
     bt.track(prep)
@@ -603,37 +647,41 @@ def extra_mixed_batch(threads):
             "vs_truth_median_mm": float(np.median(et[:, 1]))}
 
 
-def fp64_check(g, d, gt, Trel, ok, n, W, H, L, N, iters, cal, threads):
+def fp64_reference(g, d, gt, n, W, H, L, N, iters, cal, threads):
     """The FP64 oracle tracker started at the true motion for the first n
-    pairs; errors of the converged results of those pairs against it."""
+    pairs (the convergence judge): its converged poses, n x 4 x 4."""
     import threading
 
     import oracle
-    from recon_amd.synth import pose_error
     npx = W * H
     gh = g.to_array(np.uint8, 2 * n * npx).reshape(2 * n, H, W)
     dh = d.to_array(np.uint16, 2 * n * npx).reshape(2 * n, H, W)
     res = [None] * n
+    nt = max(1, threads if threads > 0 else host_cpu()[1])
+    nxt = [0]
+    lock = threading.Lock()
 
-    def work(i):
-        o = oracle.Tracker(W, H, cal, mode=oracle.PHOTOMETRIC, levels=L, iters=iters, nselect=N, dtype=np.float64)
-        o.set_reference(gh[2 * i], dh[2 * i])
-        o.set_current(gh[2 * i + 1], dh[2 * i + 1])
-        o.prepare()
-        o.set_trel(gt[i])
-        o.track()
-        res[i] = o.poses()[0]
+    def work():
+        while True:
+            with lock:
+                i = nxt[0]
+                nxt[0] += 1
+            if i >= n:
+                return
+            o = oracle.Tracker(W, H, cal, mode=oracle.PHOTOMETRIC, levels=L, iters=iters, nselect=N, dtype=np.float64)
+            o.set_reference(gh[2 * i], dh[2 * i])
+            o.set_current(gh[2 * i + 1], dh[2 * i + 1])
+            o.prepare()
+            o.set_trel(gt[i])
+            o.track()
+            res[i] = o.poses()[0]
 
-    ts = [threading.Thread(target=work, args=(i,)) for i in range(n)]
+    ts = [threading.Thread(target=work) for _ in range(min(nt, n))]
     for t in ts:
         t.start()
     for t in ts:
         t.join()
-    e = np.array([pose_error(Trel[i], res[i]) for i in range(n)])
-    sel = ok[:n]
-    return {"pairs": n, "converged": int(sel.sum()),
-            "max_deg": float(e[sel, 0].max()) if sel.any() else None,
-            "max_mm": float(e[sel, 1].max()) if sel.any() else None}
+    return np.array(res)
 
 
 if __name__ == "__main__":

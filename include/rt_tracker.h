@@ -250,6 +250,12 @@ int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_result* out
  * rt_batch_propose). out NULL: enqueue without waiting. */
 int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_result* out);
 void* rt_batch_stream(rt_batch* b);
+/* The persistent kernel's variant table (the autotuner's candidates; the
+ * environment variable RT_BATCH_VARIANT=v pins variant v): info receives
+ * {workgroup size, points per thread (unroll depth where residuals live in
+ * LDS), residuals in LDS (0/1), LDS point-cache capacity in points, waves per
+ * SIMD}. Returns RT_ERR_ARG past the end of the table. */
+int rt_batch_variant_info(int v, int info[5]);
 /* Inspection of one pair of the last prepared batch (for parity tests):
  * RT_DBG_GRAD_HIST, RT_DBG_SELECTED, RT_DBG_POINTS, RT_DBG_JACOBIAN (level),
  * RT_DBG_TREL (the pair's initial pose); layouts as for single trackers. */

@@ -802,15 +802,24 @@ static size_t track_dyn_lds(const Variant& v, int maxN) {
     return n;
 }
 
-// RZ and point-cache variants need more dynamic LDS than the default limit allows
+constexpr size_t LDS_PER_CU = 160 * 1024;  // gfx950
+constexpr int NUM_VARIANTS = (int)(sizeof(kVariants) / sizeof(kVariants[0]));
+static_assert(NUM_VARIANTS <= 64, "variant tables below hold 64 entries");
+
+// RZ and point-cache variants need more dynamic LDS than the default limit
+// allows. A variant is usable only when its static LDS plus the largest dynamic
+// LDS it may be launched with fits the CU's 160 KiB: a launch beyond that is
+// rejected and would otherwise time as ~0 ms in the autotuner.
 static bool lds_variant_ready(const Variant& v, int maxN) {
     static bool tried[64] = {}, ok[64] = {};
     int idx = (int)(&v - kVariants);
     if (!tried[idx]) {
         tried[idx] = true;
         const size_t need = (v.rz ? sizeof(float2) * RZ_MAX : 0) + sizeof(float4) * v.pc;
-        ok[idx] = hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) ==
-                  hipSuccess;
+        hipFuncAttributes fa;
+        ok[idx] = hipFuncGetAttributes(&fa, (const void*)v.fn) == hipSuccess && fa.sharedSizeBytes + need <= LDS_PER_CU &&
+                  (need == 0 || hipFuncSetAttribute((const void*)v.fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                    (int)need) == hipSuccess);
         (void)hipGetLastError();
     }
     return ok[idx] && (!v.rz || maxN <= RZ_MAX);
@@ -844,7 +853,7 @@ static int autotune(rt_batch* b) {
     hipEventCreate(&e0); hipEventCreate(&e1);
     float best = 1e30f;
     int bv = -1, bg = 0;
-    for (int v = 0; v < (int)(sizeof(kVariants) / sizeof(kVariants[0])); v++) {
+    for (int v = 0; v < NUM_VARIANTS; v++) {
         const Variant& var = kVariants[v];
         if (var.rz) {
             if (!lds_variant_ready(var, b->maxN)) continue;
@@ -858,7 +867,9 @@ static int autotune(rt_batch* b) {
         int per_cu_max = 2048 / var.bs;
         for (int k = 1; k <= 4 && k <= per_cu_max; k *= 2) {
             int grid = std::min(b->n, b->cus * k);
+            (void)hipGetLastError();
             launch_track(b, v, grid, b->stream);  // warm
+            if (hipGetLastError() != hipSuccess) break;  // launch rejected: not a candidate
             hipEventRecord(e0, b->stream);
             launch_track(b, v, grid, b->stream);
             hipEventRecord(e1, b->stream);
@@ -879,6 +890,16 @@ static int autotune(rt_batch* b) {
     if (getenv("RT_VERBOSE"))
         fprintf(stderr, "rt_batch autotune %s -> bs %d ppt %d rz %d wpe %d pc %d grid %d (%.3f ms)\n", key.c_str(),
                 kVariants[bv].bs, kVariants[bv].ppt, kVariants[bv].rz, kVariants[bv].wpe, kVariants[bv].pc, bg, best);
+    return RT_OK;
+}
+
+// The kernel variant table, for tests that force each variant through
+// RT_BATCH_VARIANT: info = {workgroup size, points per thread (unroll depth
+// for rz), residuals in LDS, point-cache capacity, waves per SIMD}.
+extern "C" int rt_batch_variant_info(int v, int info[5]) {
+    if (!info || v < 0 || v >= NUM_VARIANTS) return set_error(RT_ERR_ARG, "rt_batch_variant_info: no such variant");
+    const Variant& k = kVariants[v];
+    info[0] = k.bs; info[1] = k.ppt; info[2] = k.rz; info[3] = k.pc; info[4] = k.wpe;
     return RT_OK;
 }
 

@@ -49,7 +49,7 @@ def test_batch_matches_single_pair(robust, bins, solver, rig):
         rot, tmm = synth.pose_error(Trel[i], Ts)
         print(f"  pair {i} {W}x{H}: batch vs single {rot:.2e} deg {tmm:.2e} mm; vs truth {synth.pose_error(Trel[i], gts[i])}",
               flush=True)
-        assert np.radians(rot) < 1e-4 and tmm < 5e-2
+        assert np.radians(rot) < 1e-5 and tmm < 1e-2
         assert np.allclose(T[i] @ Trel[i], np.eye(4), atol=1e-5)
 
 

@@ -201,7 +201,12 @@ def test_track_matches_oracle(mode, solver):
     for dt, To in res.items():
         rot, tr_mm = synth.pose_error(Tdev, To)
         print(f"  device vs oracle {dt.__name__}: {rot:.2e} deg, {tr_mm:.2e} mm")
-        assert np.radians(rot) < 1e-4 and tr_mm < 5e-2
+        if dt == np.float32:
+            # the reference's arithmetic restated: agreement within 1e-5 rad / 1e-2 mm
+            assert np.radians(rot) < 1e-5 and tr_mm < 1e-2
+        else:
+            # FP64 oracle: the FP32 path's own rounding, stated per result
+            assert rot < 1e-2 and tr_mm < 0.1
     rot, tmm = synth.pose_error(Tdev, Tgt)
     print(f"  device vs ground truth: {rot:.3f} deg, {tmm:.3f} mm (start {_rot_deg(Tgt):.3f} deg)")
     assert rot < 0.1 and tmm < 5.0

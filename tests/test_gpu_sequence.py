@@ -2,10 +2,11 @@
 the CPU oracle run through the same control flow, and a 300-frame
 bi-objective sequence at 640x480 against its ground-truth trajectory.
 
-Fusion is per-pixel FP32 arithmetic in the oracle's order: bit-exact. Poses
-after Gauss-Newton are compared within 1e-4 rad / 0.05 mm per tracked frame
-(the FP32 block sums are combined in a different order than the oracle's FP64
-sums); along a sequence the global poses within 0.01 deg / 0.1 mm."""
+Fusion is per-pixel FP32 arithmetic in the oracle's order: bit-exact. Each
+frame's pose relative to its keyframe is compared within 1e-5 rad / 1e-2 mm
+(the FP32 block sums are combined in a different order than the oracle's);
+the global poses, which compose those relative poses along the sequence,
+within 1e-4 rad / 0.05 mm."""
 import ctypes
 
 import numpy as np
@@ -87,10 +88,12 @@ def test_sequence_matches_oracle_control_flow(mode):
     o.set_reference(gh[0], dh[0])
     kf = 0
     G = [np.eye(4)]
+    kfs = [0]
     for k in range(1, n):
         if k - 1 > 0 and (k - 1) % interval == 0:
             o.set_reference(gh[k - 1], dh[k - 1])
             kf = k - 1
+        kfs.append(kf)
         o.set_current(gh[k], dh[k])
         o.track()
         j = k - kf
@@ -101,7 +104,15 @@ def test_sequence_matches_oracle_control_flow(mode):
         G.append(o.poses()[3])
     G = np.array(G)
     err = np.array([synth.pose_error(P[k], G[k]) for k in range(n)])
-    assert err[:, 0].max() < 0.01 and err[:, 1].max() < 0.1, err.max(axis=0)
+    # each frame's pose relative to its keyframe (what one tracking call
+    # estimates) within 1e-5 rad / 1e-2 mm; the global poses compose those
+    # relative poses along the sequence, so their differences add up
+    rel = np.array([synth.pose_error(np.linalg.inv(P[kfs[k]]) @ P[k], np.linalg.inv(G[kfs[k]]) @ G[k])
+                    for k in range(1, n)])
+    print(f"  vs FP32 oracle: relative to the keyframe max {rel[:, 0].max():.2e} deg {rel[:, 1].max():.2e} mm; "
+          f"global max {err[:, 0].max():.2e} deg {err[:, 1].max():.2e} mm", flush=True)
+    assert np.radians(rel[:, 0].max()) < 1e-5 and rel[:, 1].max() < 1e-2, rel.max(axis=0)
+    assert np.radians(err[:, 0].max()) < 1e-4 and err[:, 1].max() < 5e-2, err.max(axis=0)
     # and it follows the true trajectory
     et = np.array([synth.pose_error(P[k], gt[k]) for k in range(n)])
     assert et[:, 0].max() < 1.0 and et[:, 1].max() < 20.0, et.max(axis=0)
