@@ -181,7 +181,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("learned", "small"), default="learned")
+    ap.add_argument("--workload", choices=("learned", "small", "single", "sequence"), default="learned",
+                    help="learned: 4096 pairs at 640x480, large motions, network proposals + batched tracking "
+                         "(the headline); small: the same batch from identity at small motions; single: one pair "
+                         "at 320x240 on the single-pair tracker; sequence: 300 frames at 640x480, bi-objective, "
+                         "keyframes every 10 frames fused with the next 5")
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--plain-pairs", type=int, default=64)
     ap.add_argument("--width", type=int, default=640)
@@ -216,6 +220,12 @@ def main():
 
     if args.tsdf:
         tsdf_bench(args)
+        return
+    if args.workload == "single":
+        single_bench(args)
+        return
+    if args.workload == "sequence":
+        sequence_bench(args)
         return
 
     from recon_amd import PHOTOMETRIC
@@ -283,6 +293,7 @@ def main():
     log(f"plain single-pair: {P} pairs in {plain_s * 1e3:.1f} ms -> {plain_fps:.1f} frames/s")
 
     # ---- batched identity start (untimed, for the convergence comparison)
+    bt.prepare_only(prepared)
     ident, _, _, _ = bt.retrack(np.tile(np.eye(4).ravel(), B), B)
 
     # ---- the timed step
@@ -411,6 +422,218 @@ def main():
         "other_workloads": extras,
     }
     print(json.dumps(out), flush=True)
+
+
+def _line(metric, value, unit, args, steps, step_s, dtype, config, **extra):
+    out = {"metric": metric, "value": round(value, 1), "unit": unit, "n_gpus": 1, "steps": steps,
+           "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": dtype, "data": "synthetic", "config": config}
+    out.update(extra)
+    print(json.dumps(out), flush=True)
+
+
+def single_bench(args):
+    """One pair at 320x240 (fx = fy = 262.5, rotation <= 2 deg, translation <=
+    30 mm) on the single-pair tracker: 3 levels x 10 Gauss-Newton iterations,
+    256-bin MAD Tukey weights, 8192 points. A step = keyframe preparation +
+    tracking of the pair from frames resident in HBM to the pose on the host.
+    Roofline: 60 B per point-iteration plus 12.3 B per level-0 pixel of
+    conversion and pyramids (15.7 MB per pair)."""
+    import oracle
+    from recon_amd import PHOTOMETRIC, synth
+    from recon_amd.hip import DeviceBuffer, Event
+    from recon_amd.tracker import PhotoTracker
+    W, H, N, iters = 320, 240, 8192, (10, 10, 10, 0)
+    g0, d0, g1, d1, Tgt = synth.make_pair(1, W, H, 2.0, 30.0)
+    cal = synth.calib_block_np(W, H)
+    bufs = [DeviceBuffer.from_array(a) for a in (g0, d0, g1, d1)]
+    tr = PhotoTracker(W, H, cal, mode=PHOTOMETRIC, n_select=N, iters=iters, mad_bins=256)
+    s = tr.L.rt_stream(tr.h)
+
+    def step():
+        tr.set_reference_points(bufs[0].ptr, bufs[1].ptr, mode=PHOTOMETRIC, force=True, on_device=True)
+        tr.set_current_image(bufs[2].ptr, bufs[3].ptr, on_device=True)
+        tr.track()
+
+    for _ in range(max(1, args.warmup) * 10):
+        step()
+    tr.synchronize()
+    steps = max(args.steps, 1) * 50
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step()
+    tr.synchronize()
+    step_s = (time.perf_counter() - t0) / steps
+    # the step's device time on the tracker's stream (HIP events)
+    e0, e1 = Event(), Event()
+    e0.record(s)
+    for _ in range(20):
+        tr.set_reference_points(bufs[0].ptr, bufs[1].ptr, mode=PHOTOMETRIC, force=True, on_device=True)
+        tr.set_current_image(bufs[2].ptr, bufs[3].ptr, on_device=True)
+        tr.L.rt_track(tr.h)
+    e1.record(s)
+    e1.synchronize()
+    dev_ms = e0.elapsed_ms(e1) / 20
+    # host frames in (PCIe inclusive), for reference
+    t1 = time.perf_counter()
+    for _ in range(20):
+        tr.set_reference_points(g0, d0, mode=PHOTOMETRIC, force=True)
+        tr.set_current_image(g1, d1)
+        tr.track()
+    host_s = (time.perf_counter() - t1) / 20
+    T = tr.trel().astype(np.float64)
+    res = {}
+    for dt in (np.float32, np.float64):
+        o = oracle.Tracker(W, H, cal, mode=oracle.PHOTOMETRIC, levels=3, iters=iters, nselect=N, mad_bins=256, dtype=dt)
+        o.set_reference(g0, d0); o.set_current(g1, d1); o.track()
+        res[dt] = o.poses()[0]
+    e32 = synth.pose_error(T, res[np.float32])
+    e64 = synth.pose_error(T, res[np.float64])
+    et = synth.pose_error(T, Tgt)
+    alg = sum(iters[:3]) * N * BYTES_PER_POINT_ITER + 12.3 * W * H
+    achieved = alg / (dev_ms * 1e-3) / 1e9
+    cpu = None
+    if args.cpu_seconds > 0:
+        cpu = cpu_sample(lambda: _oracle_pair(oracle, W, H, cal, g0, d0, g1, d1, iters, N), args.cpu_seconds,
+                         args.cpu_threads, "the pair at 320x240, FP32 oracle tracker (the reference's kernels restated "
+                                           "on the CPU), one pair per thread")
+    log(f"single pair 320x240: {1 / step_s:.0f} pairs/s (device {dev_ms:.3f} ms per pair, host frames in "
+        f"{1 / host_s:.0f} pairs/s); vs FP32 oracle {e32[0]:.2e} deg {e32[1]:.2e} mm, vs FP64 oracle {e64[0]:.2e} deg "
+        f"{e64[1]:.2e} mm, vs truth {et[0]:.3f} deg {et[1]:.2f} mm")
+    _line("tracked frames/s, one pair at 320x240 on the single-pair tracker", 1.0 / step_s, "frames/s", args, steps,
+          step_s, "f32+f64",
+          {"workload": "plain photometric tracking of one pair at 320x240 (fx = fy = 262.5), rotation <= 2 deg, "
+                       "translation <= 30 mm, 3 levels x 10 iterations, 256-bin MAD Tukey, 8192 points; frames "
+                       "resident in HBM, keyframe preparation in every step", "width": W, "height": H, "points": N},
+          host_frames_fps=round(1.0 / host_s, 1),
+          pose_error={"vs_fp32_oracle_deg": e32[0], "vs_fp32_oracle_mm": e32[1], "vs_fp64_oracle_deg": e64[0],
+                      "vs_fp64_oracle_mm": e64[1], "vs_truth_deg": et[0], "vs_truth_mm": et[1]},
+          roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes": alg,
+                    "kernel": "the whole step (keyframe preparation + 30 iterations), HIP events on the tracker's "
+                              "stream", "kernel_ms": round(dev_ms, 4)},
+          cpu_baseline=cpu)
+
+
+def _oracle_pair(oracle, W, H, cal, g0, d0, g1, d1, iters, N):
+    o = oracle.Tracker(W, H, cal, mode=oracle.PHOTOMETRIC, levels=3, iters=iters, nselect=N, mad_bins=256,
+                       dtype=np.float32)
+    o.set_reference(g0, d0); o.set_current(g1, d1); o.track()
+    return 1
+
+
+def cpu_sample(work, seconds, threads, what):
+    """Runs work() (returning the frames it tracked) on `threads` threads (0:
+    nproc) for about `seconds`; frames/s over all threads."""
+    import threading
+    model, nproc = host_cpu()
+    threads = max(1, min(threads, nproc)) if threads > 0 else nproc
+    done = [0] * threads
+    t0 = time.perf_counter()
+    stop = t0 + seconds
+
+    def run(k):
+        while time.perf_counter() < stop:
+            done[k] += work()
+
+    ts = [threading.Thread(target=run, args=(k,)) for k in range(threads)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    el = time.perf_counter() - t0
+    n = sum(done)
+    return {"value": round(n / el, 3), "unit": "frames/s", "cores": threads, "kind": "port", "cpu_model": model,
+            "nproc": nproc, "logical_cpus": os.cpu_count(), "sample": f"{n} frames: {what}; {threads} threads, "
+                                                                     f"{el:.1f} s wall"}
+
+
+def sequence_bench(args):
+    """300 frames at 640x480 (fx = fy = 525) along a seeded smooth trajectory
+    (<= 1.5 deg, <= 20 mm per frame), bi-objective (lambda 0.5) tracking with
+    61440 photometric points, 4 levels, iterations {1,3,10,5}; a keyframe every
+    10 frames, fused with the next 5. A step = the whole sequence, frames
+    resident in HBM, poses to the host at the end (rt_track_sequence).
+    Roofline: per frame and iteration 60 B per photometric point and 80 B per
+    valid geometric point of the level (the last frame's counts), plus 12.3 B
+    per pixel of conversion and pyramids and the keyframe preparation
+    amortised over its 10 frames."""
+    import oracle
+    from recon_amd import BIOBJECTIVE, synth, synth_lib
+    from recon_amd.hip import DeviceBuffer, Event, hip
+    from recon_amd.pipeline import relative_pose_error
+    from recon_amd.tracker import PhotoTracker
+    W, H, n, seed, N, iters = 640, 480, 300, 5, 61440, (1, 3, 10, 5)
+    S = synth_lib()
+    poses = synth.trajectory(seed * 13 + 5, n, 1.5, 20.0, 0, 1.0)
+    fx, fy, cx, cy = synth.pinhole(W, H)
+    npx = W * H
+    g, d = DeviceBuffer(n * npx), DeviceBuffer(n * npx * 2)
+    sc = S.rs_scene_create(ctypes.c_uint64(seed))
+    dp = ctypes.POINTER(ctypes.c_double)
+    for k in range(n):
+        T = np.ascontiguousarray(poses[k], np.float64).ravel()
+        S.rs_render_device(sc, T.ctypes.data_as(dp), W, H, fx, fy, cx, cy, ctypes.c_uint64(seed * 1000 + k), 0.01,
+                           g.ptr + k * npx, d.ptr + k * npx * 2, None)
+    hip().hipDeviceSynchronize()
+    S.rs_scene_destroy(sc)
+    cal = synth.calib_block_np(W, H)
+    tr = PhotoTracker(W, H, cal, mode=BIOBJECTIVE, n_select=N, lam=0.5, iters=iters)
+    s = tr.L.rt_stream(tr.h)
+    for _ in range(max(1, args.warmup)):
+        tr.track_sequence(g.ptr, d.ptr, 30, 10, 5)
+    steps = max(1, min(args.steps, 3))
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        P = tr.track_sequence(g.ptr, d.ptr, n, 10, 5)
+    step_s = (time.perf_counter() - t0) / steps
+    fps = n / step_s
+    e0, e1 = Event(), Event()
+    e0.record(s)
+    tr.track_sequence(g.ptr, d.ptr, n, 10, 5)
+    e1.record(s)
+    e1.synchronize()
+    dev_ms = e0.elapsed_ms(e1)
+    ng = tr.debug_read("ICP_COUNT", 0, np.int32, 4)
+    per_frame = sum(iters[3 - l] * (BYTES_PER_POINT_ITER * N + 80 * int(ng[l])) for l in range(4)) + 12.3 * W * H
+    per_frame += (W * H * 8 + 4 * 40 * N) / 10.0
+    achieved = per_frame * n / (dev_ms * 1e-3) / 1e9
+    et = np.array([synth.pose_error(P[k], poses[k]) for k in range(n)])
+    rpe_mm, rpe_deg = relative_pose_error(P, poses, 100)
+    cpu = None
+    if args.cpu_seconds > 0:
+        gh = g.to_array(np.uint8, 31 * npx).reshape(31, H, W)
+        dh = d.to_array(np.uint16, 31 * npx).reshape(31, H, W)
+
+        def work():
+            # frames 1..10 of the sequence against keyframe 0, fused as on the GPU
+            o = oracle.Tracker(W, H, cal, mode=oracle.BIOBJECTIVE, levels=4, iters=iters, nselect=N, lam=0.5,
+                               dtype=np.float32)
+            o.set_reference(gh[0], dh[0])
+            for k in range(1, 3):
+                o.set_current(gh[k], dh[k]); o.track(); o.fuse_current()
+            return 2
+        cpu = cpu_sample(work, args.cpu_seconds, args.cpu_threads,
+                         "frames 1-2 of the sequence against keyframe 0 with fusion, bi-objective, 61440 points, FP32 "
+                         "oracle tracker (the reference's kernels restated on the CPU), one sequence per thread")
+    log(f"sequence 640x480: {fps:.1f} frames/s ({dev_ms / n:.3f} ms per frame on the device); {achieved:.0f} GB/s "
+        f"algorithmic; final error {et[-1, 0]:.3f} deg {et[-1, 1]:.2f} mm; RPE(100) {rpe_mm:.2f} mm {rpe_deg:.3f} deg; "
+        f"ICP points per level {list(map(int, ng))}")
+    _line("tracked frames/s of a 300-frame bi-objective sequence at 640x480", fps, "frames/s", args, steps, step_s,
+          "f32+f64",
+          {"workload": "bi-objective (lambda 0.5) keyframe tracking of a seeded 300-frame sequence at 640x480, "
+                       "fx = fy = 525, <= 1.5 deg / 20 mm per frame, 61440 photometric points, 4 levels, iterations "
+                       "[1, 3, 10, 5], keyframe every 10 frames fused with the next 5; frames resident in HBM",
+           "frames": n, "width": W, "height": H, "points": N},
+          drift={"final_deg": et[-1, 0], "final_mm": et[-1, 1], "rpe100_mm": rpe_mm, "rpe100_deg": rpe_deg,
+                 "drift_mm_per_100_frames": et[-1, 1] / (n / 100.0), "max_deg": float(et[:, 0].max()),
+                 "max_mm": float(et[:, 1].max())},
+          roofline={"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "algorithmic_bytes_per_frame": per_frame,
+                    "icp_points_per_level": [int(x) for x in ng],
+                    "kernel": "the whole sequence (every launch of rt_track_sequence), HIP events on the tracker's "
+                              "stream", "kernel_ms": round(dev_ms, 3)},
+          cpu_baseline=cpu)
 
 
 def tsdf_bench(args):

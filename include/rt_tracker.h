@@ -261,6 +261,18 @@ int rt_batch_variant_info(int v, int info[5]);
  * RT_DBG_TREL (the pair's initial pose); layouts as for single trackers. */
 int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, void* host, size_t bytes);
 
+/* Inspection for bounds checks on the host (no device is touched): the pair
+ * descriptors rt_batch_prepare builds for n pairs in a batch of max_width x
+ * max_height, as byte offsets from the batch's arena. fields: 24 int64 per
+ * pair: W, H, levels, points, simple-calibration flag, offsets of the gray
+ * pyramid levels 1-3 of the keyframe and of the current frame (-1 where the
+ * level is absent), of the gradient bins, gradient histogram, selection,
+ * points and the point records of levels 0-3, the pair's arena slot start
+ * and the slot size, then 3 zeros. order: the work order (n ints); maxes:
+ * the largest width, height and point count. */
+int rt_batch_layout(const rt_params* params, int max_width, int max_height, int n, const rt_pair* pairs,
+                    int64_t* fields, int* order, int maxes[3]);
+
 /* ------------------------------------------------------------------ inspection
  * Read an internal device buffer into host memory, for parity tests.
  * what: RT_DBG_*; level: pyramid level where applicable. */

@@ -640,7 +640,7 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
                     SolveCfg sc;
                     sc.solver = cfg.solver; sc.mode = 1; sc.lambda = 0.0f; sc.scale_out = cfg.scale_out;
                     sc.robust = cfg.robust; sc.mad_bins = cfg.mad_bins; sc.scale_in = 1e-3f;
-                    const bool ok = solve_update_wave(sys, T, sc, N, ws);
+                    const bool ok = solve_update_wave(sys, T, sc, N, 1, ws);
                     if (t == 0) {
                         if (!ok) s_ok = 0;
                         rms_last = sys[28] > 0.0 ? (float)sqrt(sys[27] / sys[28]) : 0.0f;
@@ -714,6 +714,29 @@ struct rt_batch {
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Device memory a batch's descriptors point into: per-pair arenas of
+// pair_bytes (sized for max_w x max_h and nsel points) and the per-pair tables.
+struct BatchMem {
+    char* arena;
+    size_t pair_bytes;
+    int max_w, max_h;
+    size_t nsel;
+    float* calib;     // 96 per pair
+    float* trel_in;   // 16 per pair
+    float* trel_out;  // 16 per pair
+    float* rms;
+    int* status;
+};
+static size_t pair_arena_bytes(int max_w, int max_h, size_t N) {
+    const size_t npx = (size_t)max_w * max_h;
+    size_t per = 0;  // inputs are borrowed device pointers; the arena holds derived data only
+    per += 2 * align256(npx * 4 / 3 + 64);      // two gray pyramids, levels 1..3
+    per += align256(npx);                       // gradient-magnitude bins
+    per += align256(256 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(jac_bytes((int)N));
+    return per;
+}
+
+
 extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, const rt_params* params,
                                rt_batch** out) {
     if (!out || max_pairs < 1 || max_w < 32 || max_h < 32) return set_error(RT_ERR_ARG, "rt_batch_create: bad argument");
@@ -728,14 +751,8 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) b->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return set_error(RT_ERR_HIP, "stream"); }
-    size_t npx = (size_t)max_w * max_h;
-    size_t N = (size_t)b->prm.n_select;
-    size_t per = 0;  // inputs are borrowed device pointers; the arena holds derived data only
-    per += 2 * align256(npx * 4 / 3 + 64);      // two gray pyramids, levels 1..3
-    per += align256(npx);                       // gradient-magnitude bins
-    per += align256(256 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(jac_bytes((int)N));
-    b->pair_bytes = per;
-    size_t total = per * (size_t)max_pairs;
+    b->pair_bytes = pair_arena_bytes(max_w, max_h, (size_t)b->prm.n_select);
+    size_t total = b->pair_bytes * (size_t)max_pairs;
     void* q = nullptr;
     if (hipMalloc(&q, total) != hipSuccess) { hipStreamDestroy(b->stream); delete b; return set_error(RT_ERR_NOMEM, "batch arena"); }
     b->arena = (char*)q; b->allocs.push_back(q);
@@ -773,24 +790,38 @@ extern "C" void rt_batch_destroy(rt_batch* b) {
 
 extern "C" void* rt_batch_stream(rt_batch* b) { return b ? (void*)b->stream : nullptr; }
 
-static void launch_prep(rt_batch* b, hipStream_t s) {
-    int n = b->n;
-    dim3 g64((b->maxW + 63) / 64, (b->maxH + 63) / 64, n);
-    kb_pyramid<<<g64, 256, 0, s>>>(b->desc_dev, 0);
-    kb_pyramid<<<g64, 256, 0, s>>>(b->desc_dev, 1);
-    dim3 gb((b->maxW + 63) / 64, (b->maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
-    kb_bins<<<gb, 256, 0, s>>>(b->desc_dev);
-    kb_select<<<n, SEL_BS, 0, s>>>(b->desc_dev);
-    kb_ref_points<<<dim3((b->maxN + 255) / 256, n), 256, 0, s>>>(b->desc_dev);
+// Where the launches of one batch find their tables: the batch's own buffers,
+// or the fixed scratch of rt_batch_track_noalloc.
+struct BatchView {
+    const PairDesc* desc;
+    const int* order;
+    int* queue;
+    int n, maxW, maxH, maxN;
+    int solver, robust, mad_bins;
+};
+static BatchView view_of(const rt_batch* b) {
+    return BatchView{b->desc_dev, b->order_dev, b->queue, b->n, b->maxW, b->maxH, b->maxN,
+                     b->prm.solver, b->prm.robust, b->prm.mad_bins};
 }
 
-static TrackCfg track_cfg(const rt_batch* b, const Variant& v) {
+static void launch_prep(const BatchView& v, hipStream_t s) {
+    const int n = v.n;
+    dim3 g64((v.maxW + 63) / 64, (v.maxH + 63) / 64, n);
+    kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 0);
+    kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 1);
+    dim3 gb((v.maxW + 63) / 64, (v.maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
+    kb_bins<<<gb, 256, 0, s>>>(v.desc);
+    kb_select<<<n, SEL_BS, 0, s>>>(v.desc);
+    kb_ref_points<<<dim3((v.maxN + 255) / 256, n), 256, 0, s>>>(v.desc);
+}
+
+static TrackCfg track_cfg(const BatchView& bv, const Variant& v) {
     TrackCfg c;
     c.pc_cap = v.pc;
-    c.rz_len = b->maxN;
-    c.solver = b->prm.solver;
-    c.robust = b->prm.robust;
-    c.mad_bins = b->prm.mad_bins;
+    c.rz_len = bv.maxN;
+    c.solver = bv.solver;
+    c.robust = bv.robust;
+    c.mad_bins = bv.mad_bins;
     c.scale_out = 1e3f;
     return c;
 }
@@ -825,11 +856,12 @@ static bool lds_variant_ready(const Variant& v, int maxN) {
     return ok[idx] && (!v.rz || maxN <= RZ_MAX);
 }
 
-static void launch_track(rt_batch* b, int variant, int grid, hipStream_t s) {
-    kb_reset_queue<<<1, 64, 0, s>>>(b->queue);
+static void launch_track(const BatchView& bv, int variant, int grid, hipStream_t s) {
+    kb_reset_queue<<<1, 64, 0, s>>>(bv.queue);
     const Variant& v = kVariants[variant];
-    v.fn<<<grid, v.bs, track_dyn_lds(v, b->maxN), s>>>(b->desc_dev, b->order_dev, b->n, b->queue, track_cfg(b, v));
+    v.fn<<<grid, v.bs, track_dyn_lds(v, bv.maxN), s>>>(bv.desc, bv.order, bv.n, bv.queue, track_cfg(bv, v));
 }
+static void launch_track(rt_batch* b, int variant, int grid, hipStream_t s) { launch_track(view_of(b), variant, grid, s); }
 
 static std::string tune_key(const rt_batch* b) {
     // batch size bucket and the size mix decide the best configuration
@@ -906,7 +938,7 @@ extern "C" int rt_batch_variant_info(int v, int info[5]) {
 static int capture(rt_batch* b, hipGraphExec_t* out, bool prep) {
     hipGraph_t g;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return set_error(RT_ERR_HIP, "begin capture");
-    if (prep) launch_prep(b, b->stream);
+    if (prep) launch_prep(view_of(b), b->stream);
     else launch_track(b, b->variant, b->grid, b->stream);
     if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return set_error(RT_ERR_HIP, "end capture");
     if (*out) hipGraphExecDestroy(*out);
@@ -951,20 +983,20 @@ static int run_track(rt_batch* b, rt_pair_result* out) {
     return out ? fetch_results(b, out) : RT_OK;
 }
 
-extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
-    if (!b || !pairs || n < 1 || n > b->max_pairs) return set_error(RT_ERR_ARG, "rt_batch_prepare: bad argument");
-    if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
-    std::vector<PairDesc> desc(n);
-    std::vector<float> calib(96 * (size_t)n);
-    std::vector<float> trel(16 * (size_t)n);
-    int maxW = 0, maxH = 0, maxN = 0;
+// Checks the n pairs against the parameters and the memory's limits and fills
+// their descriptors (desc), calibration blocks (calib, 96 per pair), initial
+// poses (trel, 16 per pair) and the work order (decreasing cost); returns the
+// largest width, height and point count.
+static int build_batch(const rt_params& prm, const BatchMem& m, int n, const rt_pair* pairs, PairDesc* desc,
+                       float* calib, float* trel, int* order, int& maxW, int& maxH, int& maxN) {
+    maxW = maxH = maxN = 0;
     for (int i = 0; i < n; i++) {
         const rt_pair& p = pairs[i];
-        rt_params q = b->prm;
+        const rt_params& q = prm;
         int L = q.levels > 0 ? q.levels : (p.width == 640 ? 4 : 3);
-        int m = std::max(4, 1 << (L - 1));
+        int mm = std::max(4, 1 << (L - 1));
         if (!p.ref_gray || !p.ref_depth || !p.cur_gray || !p.cur_depth || !p.calib) return set_error(RT_ERR_ARG, "null pair input");
-        if (p.width > b->max_w || p.height > b->max_h || p.width % (2 * m) || p.height % m || p.width < 32 || p.height < 32)
+        if (p.width > m.max_w || p.height > m.max_h || p.width % (2 * mm) || p.height % mm || p.width < 32 || p.height < 32)
             return set_error(RT_ERR_ARG, "pair size not supported by this batch");
         if ((p.width * p.height) / 512 > 1024) return set_error(RT_ERR_ARG, "pair too large");
         int N = q.n_select;
@@ -973,11 +1005,11 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
         memset(&d, 0, sizeof(d));
         d.W = p.width; d.H = p.height; d.L = L; d.N = N;
         for (int k = 0; k < 4; k++) d.iters[k] = q.iters[k];
-        d.calib = b->calib_dev + 96 * (size_t)i;
+        d.calib = m.calib + 96 * (size_t)i;
         memcpy(&calib[96 * (size_t)i], p.calib, 96 * 4);
         d.simple = calib_simple(p.calib) ? 1 : 0;
         d.g8[0] = p.ref_gray; d.d16[0] = p.ref_depth; d.g8[1] = p.cur_gray; d.d16[1] = p.cur_depth;
-        char* base = b->arena + b->pair_bytes * (size_t)i;
+        char* base = m.arena + m.pair_bytes * (size_t)i;
         size_t npx = (size_t)p.width * p.height, off = 0;
         for (int f = 0; f < 2; f++) {
             size_t lo = 0;
@@ -986,27 +1018,41 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
                 d.gray[f][l] = l < L ? (float*)(base + off + lo) : nullptr;
                 lo += (npx >> (2 * l)) * 4;
             }
-            off += align256((size_t)b->max_w * b->max_h * 4 / 3 + 64);
+            off += align256((size_t)m.max_w * m.max_h * 4 / 3 + 64);
         }
         d.z[0] = d.z[1] = nullptr;   // depth is read from the 16-bit input
-        d.bins = (uint8_t*)(base + off); off += align256((size_t)b->max_w * b->max_h);
+        d.bins = (uint8_t*)(base + off); off += align256((size_t)m.max_w * m.max_h);
         d.ghist = (unsigned*)(base + off); off += align256(256 * 4);
-        size_t NN = (size_t)b->prm.n_select;
+        const size_t NN = m.nsel;
         d.sel = (int*)(base + off); off += align256(NN * 4);
         d.pts = (float4*)(base + off); off += align256(NN * 16);
         for (int l = 0; l < 4; l++) { d.jac[l] = (float4*)(base + off); off += align256(jac_bytes((int)NN)); }
-        d.trel_in = b->trel_in + 16 * (size_t)i;
-        d.trel_out = b->trel_out + 16 * (size_t)i;
-        d.rms = b->rms + i;
-        d.status = b->status + i;
+        d.trel_in = m.trel_in + 16 * (size_t)i;
+        d.trel_out = m.trel_out + 16 * (size_t)i;
+        d.rms = m.rms + i;
+        d.status = m.status + i;
         for (int k = 0; k < 16; k++) trel[16 * (size_t)i + k] = p.init_T ? (float)p.init_T[k] : ((k % 5 == 0) ? 1.0f : 0.0f);
         maxW = std::max(maxW, d.W); maxH = std::max(maxH, d.H); maxN = std::max(maxN, d.N);
     }
     // work queue order: decreasing cost (points x iterations), ties by index
-    std::vector<int> order(n);
     for (int i = 0; i < n; i++) order[i] = i;
     auto cost = [&](int i) { long c = 0; for (int k = 0; k < desc[i].L; k++) c += desc[i].iters[k]; return c * desc[i].N + (long)desc[i].W * desc[i].H / 8; };
-    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cost(x) > cost(y); });
+    std::stable_sort(order, order + n, [&](int x, int y) { return cost(x) > cost(y); });
+    return RT_OK;
+}
+
+extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
+    if (!b || !pairs || n < 1 || n > b->max_pairs) return set_error(RT_ERR_ARG, "rt_batch_prepare: bad argument");
+    if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    std::vector<PairDesc> desc(n);
+    std::vector<float> calib(96 * (size_t)n);
+    std::vector<float> trel(16 * (size_t)n);
+    std::vector<int> order(n);
+    const BatchMem m{b->arena, b->pair_bytes, b->max_w, b->max_h, (size_t)b->prm.n_select,
+                     b->calib_dev, b->trel_in, b->trel_out, b->rms, b->status};
+    int maxW = 0, maxH = 0, maxN = 0;
+    int rc = build_batch(b->prm, m, n, pairs, desc.data(), calib.data(), trel.data(), order.data(), maxW, maxH, maxN);
+    if (rc) return rc;
     if (!b->st_desc.upload(b->desc_dev, desc.data(), sizeof(PairDesc) * n, b->stream) ||
         !b->st_calib.upload(b->calib_dev, calib.data(), calib.size() * 4, b->stream) ||
         !b->st_order.upload(b->order_dev, order.data(), 4 * (size_t)n, b->stream) ||
@@ -1015,7 +1061,7 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
     bool changed = b->n != n || b->maxW != maxW || b->maxH != maxH || b->maxN != maxN;
     b->n = n; b->desc = desc; b->maxW = maxW; b->maxH = maxH; b->maxN = maxN;
     if (changed || !b->graph_prep) {
-        int rc = capture(b, &b->graph_prep, true);
+        rc = capture(b, &b->graph_prep, true);
         if (rc) return rc;
     }
     if (hipGraphLaunch(b->graph_prep, b->stream) != hipSuccess) return set_error(RT_ERR_HIP, "graph launch");
@@ -1077,4 +1123,38 @@ extern "C" int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_resul
             return set_error(RT_ERR_HIP, "pose upload");
     }
     return run_track(b, out);
+}
+
+// ------------------------------------------------------------------ layout inspection
+// The descriptors rt_batch_prepare would build for n pairs in a batch of
+// max_width x max_height, as offsets from the batch's arena (host code only:
+// no device is touched), for bounds checks of the launches on the CPU. Per pair
+// RT_LAYOUT_FIELDS int64 values: W, H, L, N, simple, then the byte offsets of
+// gray[0][1..3], gray[1][1..3], bins, ghist, sel, pts, jac[0..3] from the
+// arena, the pair's arena slot start, and the slot size; then the work order.
+extern "C" int rt_batch_layout(const rt_params* params, int max_w, int max_h, int n, const rt_pair* pairs,
+                               int64_t* fields, int* order, int maxes[3]) {
+    if (!pairs || !fields || !order || !maxes || n < 1) return set_error(RT_ERR_ARG, "rt_batch_layout: bad argument");
+    rt_params prm;
+    if (params) prm = *params; else rt_default_params(&prm);
+    // a nominal arena address: only offsets from it are reported
+    char* const arena = reinterpret_cast<char*>((uintptr_t)1 << 40);
+    const size_t pb = pair_arena_bytes(max_w, max_h, (size_t)prm.n_select);
+    const BatchMem m{arena, pb, max_w, max_h, (size_t)prm.n_select, nullptr, nullptr, nullptr, nullptr, nullptr};
+    std::vector<PairDesc> desc(n);
+    std::vector<float> calib(96 * (size_t)n), trel(16 * (size_t)n);
+    int rc = build_batch(prm, m, n, pairs, desc.data(), calib.data(), trel.data(), order, maxes[0], maxes[1], maxes[2]);
+    if (rc) return rc;
+    auto off = [&](const void* p) { return p ? (int64_t)((const char*)p - arena) : (int64_t)-1; };
+    for (int i = 0; i < n; i++) {
+        const PairDesc& d = desc[i];
+        int64_t* f = fields + (size_t)i * 24;
+        f[0] = d.W; f[1] = d.H; f[2] = d.L; f[3] = d.N; f[4] = d.simple;
+        for (int l = 1; l < 4; l++) { f[4 + l] = off(d.gray[0][l]); f[7 + l] = off(d.gray[1][l]); }
+        f[11] = off(d.bins); f[12] = off(d.ghist); f[13] = off(d.sel); f[14] = off(d.pts);
+        for (int l = 0; l < 4; l++) f[15 + l] = off(d.jac[l]);
+        f[19] = (int64_t)(pb * (size_t)i); f[20] = (int64_t)pb;
+        f[21] = f[22] = f[23] = 0;
+    }
+    return RT_OK;
 }

@@ -160,28 +160,50 @@ void launch_prepare_photo_reference(const Dev& d, hipStream_t s) {
                                                     d.jac[1], d.jac[2], d.jac[3]);
 }
 
-// Reduce the 27 per-thread FP32 sums of a 256-thread workgroup (wave tree, then
-// the 4 waves in order) into part[group*27 + k].
-__device__ __forceinline__ void reduce27(float* acc, float* part) {
+// Partial sums of the normal equations: FP32 per workgroup, term-major
+// (part[k * PART_STRIDE + group]) so that the combining workgroup reads each
+// term's partials contiguously.
+constexpr int PART_STRIDE = 1024;  // >= the largest grid of a system pass
+
+// Reduce the 27 per-thread FP32 sums of a 256-thread workgroup (per wave by
+// halving exchanges, then the 4 waves in order) into part[k * PART_STRIDE + group].
+__device__ __forceinline__ void reduce27(const float* acc, float* part) {
     __shared__ float wsum[4][27];
-    int wave = threadIdx.x >> 6, lane = lane_id();
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    float v32[32];
 #pragma unroll
-    for (int k = 0; k < 27; k++) {
-        float v = wave_sum(acc[k]);
-        if (lane == 0) wsum[wave][k] = v;
-    }
+    for (int k = 0; k < 32; k++) v32[k] = k < 27 ? acc[k] : 0.0f;
+    const float tot = wave_sum32(v32);
+    if ((lane & 1) == 0 && (lane >> 1) < 27) wsum[wave][lane >> 1] = tot;
     __syncthreads();
     if (threadIdx.x < 27) {
-        int k = threadIdx.x;
-        part[(size_t)blockIdx.x * 27 + k] = ((wsum[0][k] + wsum[1][k]) + wsum[2][k]) + wsum[3][k];
+        const int k = threadIdx.x;
+        part[(size_t)k * PART_STRIDE + blockIdx.x] = ((wsum[0][k] + wsum[1][k]) + wsum[2][k]) + wsum[3][k];
     }
 }
-// Last workgroup: FP64 sum over groups in group order.
+// Last workgroup (256 threads): the FP64 sum over groups of every term. Wave w
+// takes terms w, w + 4, ...; lane l sums groups l, l + 64, ... in order, then
+// the lanes' sums are added by a fixed exchange tree (deterministic).
 __device__ __forceinline__ void combine27(const float* part, int groups, double* out) {
-    if (threadIdx.x < 27) {
-        double s = 0.0;
-        for (int g = 0; g < groups; g++) s += (double)part[(size_t)g * 27 + threadIdx.x];
-        out[threadIdx.x] = s;
+    const int wave = threadIdx.x >> 6, lane = lane_id();
+    constexpr int KW = 7;  // terms per wave (4 x 7 >= 27)
+    double s[KW];
+#pragma unroll
+    for (int j = 0; j < KW; j++) s[j] = 0.0;
+    for (int g = lane; g < groups; g += 64) {
+#pragma unroll
+        for (int j = 0; j < KW; j++) {
+            const int k = wave + 4 * j;
+            if (k < 27) s[j] += (double)part[(size_t)k * PART_STRIDE + g];
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < KW; j++) {
+        double v = s[j];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+        const int k = wave + 4 * j;
+        if (lane == 0 && k < 27) out[k] = v;
     }
 }
 
@@ -262,9 +284,10 @@ __global__ __launch_bounds__(256) void k_photo_sys(const float2* __restrict__ re
     reduce27(acc, part);
     if (last_group(ticket)) {
         combine27(part, gridDim.x, sys);
+        __threadfence_block();
         __syncthreads();
         __shared__ double ws[SOLVE_WS];
-        if (t == 0 && solve_here) solve_update(sys, trel, cfg, N, 1, ws);
+        if ((t >> 6) == 0 && solve_here) solve_update_wave(sys, trel, cfg, N, 1, ws);
         // keep the histogram of this iteration (inspection) and clear it for the next
         madhist[256 + t] = madhist[t];
         madhist[t] = 0u;
@@ -676,9 +699,10 @@ __global__ __launch_bounds__(256) void k_icp_sys(const float4* __restrict__ rp, 
     reduce27(acc, part);
     if (last_group(ticket)) {
         combine27(part, gridDim.x, sys + 27);
+        __threadfence_block();
         __syncthreads();
         __shared__ double ws[SOLVE_WS];
-        if (threadIdx.x == 0 && solve_here && n > 0) solve_update(sys, trel, cfg, N1, n, ws);
+        if ((threadIdx.x >> 6) == 0 && solve_here && n > 0) solve_update_wave(sys, trel, cfg, N1, n, ws);
         if (threadIdx.x == 0) reset_ticket(ticket);
     }
 }

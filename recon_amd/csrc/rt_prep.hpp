@@ -585,30 +585,44 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     const int hi = __builtin_amdgcn_readlane(__double2hiint(v), src);
     return __hiloint2double(hi, lo);
 }
-// solve_update for a photometric system with the Cholesky solver, run by the
-// 64 lanes of one wave: lane r < 6 holds row r of A and of L in registers, the
-// factorisation computes a column's entries of all rows at once, and the
-// substitutions exchange values by lane reads instead of a workspace. Every
-// entry is computed by the same operations in the same order as in
-// solve_cholesky, so the result is the same bit for bit; the exponential
-// update runs on lane 0 as in solve_update. Other solvers and modes run
-// solve_update on lane 0. The return value is meaningful in lane 0.
-__device__ bool solve_update_wave(const double* sys, float* trel, const SolveCfg& cfg, int N1, double* w) {
+// solve_update with the Cholesky solver, run by the 64 lanes of one wave: lane
+// r < 6 holds row r of A and of L in registers, the factorisation computes a
+// column's entries of all rows at once, and the substitutions exchange values
+// by lane reads instead of a workspace. The system is assembled as in
+// solve_update (photometric terms, geometric terms, or their linear fuse with
+// the same operations), and every entry is computed by the same operations in
+// the same order as in solve_cholesky, so the result is the same bit for bit;
+// the exponential update runs in every lane's registers and lane 0 stores the
+// pose. The CG solver runs solve_update on lane 0. The return value is
+// meaningful in lane 0.
+__device__ bool solve_update_wave(const double* sys, float* trel, const SolveCfg& cfg, int N1, int N2, double* w) {
     const int lane = lane_id();
-    if (cfg.solver != 0 || cfg.mode != 1) {
+    if (cfg.solver != 0) {
         bool ok = true;
-        if (lane == 0) ok = solve_update(sys, trel, cfg, N1, 1, w);
+        if (lane == 0) ok = solve_update(sys, trel, cfg, N1, N2, w);
         return ok;
     }
     const int r = lane < 6 ? lane : 5;
     // row r of A and b_r from the 27 packed terms (unpack27's order)
+    const double* s0 = cfg.mode == 0 ? sys + 27 : sys;
     double Ar[6];
 #pragma unroll
     for (int c = 0; c < 6; c++) {
         const int lo = r < c ? r : c, hi = r < c ? c : r;
-        Ar[c] = sys[lo * 6 - lo * (lo - 1) / 2 + (hi - lo)];
+        Ar[c] = s0[lo * 6 - lo * (lo - 1) / 2 + (hi - lo)];
     }
-    const double br = sys[21 + r];
+    double br = s0[21 + r];
+    if (cfg.mode == 2) {
+        // linearFuse (warp.cu:1498-1509) as in solve_update
+        const double s1 = (1.0 - (double)cfg.lambda) / (double)N1, s2 = (double)cfg.lambda / (double)N2;
+        const double* sg = sys + 27;
+#pragma unroll
+        for (int c = 0; c < 6; c++) {
+            const int lo = r < c ? r : c, hi = r < c ? c : r;
+            Ar[c] = s1 * Ar[c] + s2 * sg[lo * 6 - lo * (lo - 1) / 2 + (hi - lo)];
+        }
+        br = s1 * br + s2 * sg[21 + r];
+    }
     double Lr[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     bool ok = true;
 #pragma unroll

@@ -1,0 +1,238 @@
+"""Host-side bounds model of the batched path, on the CPU.
+
+The library's own descriptor builder (rt_batch_layout: the code rt_batch_prepare
+runs, reporting offsets instead of touching a device) places every pair's
+buffers in the arena; this model replays the launch grids of the preparation
+and tracking launches (launch_prep, launch_track in rt_batch.hip) for those
+descriptors and forms every index the kernels form, from their code, asserting
+that each lies inside its buffer. It runs on the 256-pair mixed-size batch of
+the mixed-size workload (160x120 .. 640x480, 8192 points), the 4096-pair
+640x480 batch of the headline and small mixed batches."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from recon_amd import RtPair, lib
+from recon_amd.tracker import default_params
+
+SIZES = [(160, 120), (320, 240), (480, 360), (640, 480)]
+FIELDS = 24
+BINS_TILE_H = 16
+NW_BINS = ((64 + 8) // 4) * (BINS_TILE_H + 4)   # words of the kb_bins tile (TWW * TH)
+
+
+def _align(x):
+    return (x + 255) & ~255
+
+
+def _layout(sizes, N, max_w=640, max_h=480):
+    n = len(sizes)
+    arr = (RtPair * n)()
+    cal = np.zeros(96, np.float32)
+    cal[43], cal[45], cal[47], cal[48], cal[62], cal[63] = -525.0, 319.5, 525.0, 239.5, 500.0, 4000.0
+    cal[26:42] = np.eye(4, dtype=np.float32).ravel()
+    cal[64:80] = np.eye(4, dtype=np.float32).ravel()
+    cal[0:9] = cal[43:52]
+    for i, (W, H) in enumerate(sizes):
+        # frame pointers are never dereferenced by the layout: distinct dummies
+        arr[i].ref_gray, arr[i].ref_depth = 0x1000 + 16 * i, 0x2000 + 16 * i
+        arr[i].cur_gray, arr[i].cur_depth = 0x3000 + 16 * i, 0x4000 + 16 * i
+        arr[i].width, arr[i].height = W, H
+        arr[i].calib = cal.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    f = np.zeros(n * FIELDS, np.int64)
+    order = np.zeros(n, np.int32)
+    maxes = np.zeros(3, np.int32)
+    prm = default_params(n_select=N)
+    rc = lib().rt_batch_layout(ctypes.byref(prm), max_w, max_h, n, arr, f.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                               order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                               maxes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    assert rc == 0, lib().rt_last_error()
+    return f.reshape(n, FIELDS), order, maxes
+
+
+def _buffers(p, max_w, max_h):
+    """{name: (offset, bytes)} of one pair's arena buffers, sized by what the
+    kernels may address: the level images by the pair's own size, the other
+    buffers by their slots."""
+    W, H, L, N = (int(x) for x in p[:4])
+    b = {}
+    for f in range(2):
+        for l in range(1, 4):
+            o = int(p[4 + l + 3 * f])
+            if l < L:
+                b[f"gray{f}_{l}"] = (o, (W >> l) * (H >> l) * 4)
+            else:
+                assert o == -1
+    b["bins"] = (int(p[11]), W * H)
+    b["ghist"] = (int(p[12]), 256 * 4)
+    b["sel"] = (int(p[13]), N * 4)
+    b["pts"] = (int(p[14]), N * 16)
+    for l in range(4):
+        b[f"jac{l}"] = (int(p[15 + l]), N * 28)
+    return b
+
+
+def _check_layout(f, order, maxes, max_w, max_h, N):
+    n = len(f)
+    slot = int(f[0, 20])
+    npx = max_w * max_h
+    assert slot == 2 * _align(npx * 4 // 3 + 64) + _align(npx) + _align(1024) + _align(N * 4) + _align(N * 16) + \
+        4 * _align(N * 28)
+    assert sorted(order.tolist()) == list(range(n))         # the work queue hands out every pair once
+    assert maxes[0] == f[:, 0].max() and maxes[1] == f[:, 1].max() and maxes[2] == f[:, 3].max()
+    for i in range(n):
+        p = f[i]
+        assert p[19] == i * slot and p[20] == slot
+        bufs = sorted(_buffers(p, max_w, max_h).values())
+        for (o, sz) in bufs:
+            assert p[19] <= o and o + sz <= p[19] + slot, (i, o, sz)
+        for (o1, s1), (o2, s2) in zip(bufs, bufs[1:]):
+            assert o1 + s1 <= o2, (i, o1, s1, o2)                # disjoint
+
+
+def _kb_pyramid(W, H, L, maxW, maxH):
+    """kb_pyramid / convert_pyramid_block<3>: 16x16 threads of 4x4 pixels per
+    64x64 block; largest element index of each access, per buffer."""
+    bx, by, t = np.meshgrid(np.arange((maxW + 63) // 64), np.arange((maxH + 63) // 64), np.arange(256), indexing="ij")
+    live = (bx * 64 < W) & (by * 64 < H)
+    tx, ty = t & 15, t >> 4
+    x0, y0 = (bx * 16 + tx) * 4, (by * 16 + ty) * 4
+    inside = live & (x0 < W) & (y0 < H)
+    x0, y0 = x0[inside], y0[inside]
+    out = {"g8": ((y0 + 3) * W + x0 + 3).max()}                  # uchar4 rows y0..y0+3
+    if L > 1:
+        out["gray_1"] = ((y0 // 2 + 1) * (W >> 1) + x0 // 2 + 1).max()
+    if L > 2:
+        out["gray_2"] = ((y0 // 4) * (W >> 2) + x0 // 4).max()
+    if L > 3:
+        ev = ((t & 15)[inside] % 2 == 0) & ((t >> 4)[inside] % 2 == 0)
+        out["gray_3"] = ((y0[ev] // 8) * (W >> 3) + x0[ev] // 8).max()
+        assert ((t & 15)[inside][ev] + 1).max() <= 15 and ((t >> 4)[inside][ev] + 1).max() <= 15  # l2tile[16][17]
+    return out
+
+
+def _kb_bins(W, H, maxW, maxH):
+    """kb_bins: 64 x 16 tiles, lane = column, 4 rows per wave; the clamped
+    depth and gray-word loads, the bin writes and the LDS tile."""
+    bx, by = np.meshgrid(np.arange((maxW + 63) // 64), np.arange((maxH + BINS_TILE_H - 1) // BINS_TILE_H),
+                         indexing="ij")
+    live = (bx * 64 < W) & (by * BINS_TILE_H < H)
+    bx, by = bx[live], by[live]
+    lane = np.arange(64)
+    x = (bx[:, None] * 64 + lane[None, :]).ravel()
+    yb = by[:, None].repeat(64, 1).ravel() * BINS_TILE_H
+    out = {}
+    xc = np.minimum(x, W - 1)
+    ys = np.concatenate([yb + r for r in range(BINS_TILE_H)])
+    out["d16"] = (np.tile(xc, BINS_TILE_H) + np.minimum(ys, H - 1) * W).max()
+    e = np.arange(NW_BINS)
+    gx = (bx[:, None] * 64 - 4) + 4 * (e[None, :] % ((64 + 8) // 4))
+    gy = (by[:, None] * BINS_TILE_H - 2) + e[None, :] // ((64 + 8) // 4)
+    o = np.clip(gx, 0, W - 4) + np.clip(gy, 0, H - 1) * W
+    out["g8_words"] = (o + 3).max()
+    xs = np.tile(x, BINS_TILE_H)
+    keep = (xs < W) & (ys < H)
+    out["bins"] = (xs[keep] + ys[keep] * W).max()
+    # taps of the gradient: in_bounds keeps the projected point in
+    # [10, W - 10) x [10, H - 10); the taps are xdi - 1 .. xdi + 2, ydi - 1 .. ydi + 2
+    out["gray_taps"] = (H - 11 + 2) * W + (W - 11 + 2)
+    return out
+
+
+def _kb_select(W, H, N):
+    npix = W * H
+    Pn = npix // 512
+    assert Pn >= 1 and (Pn * 512) <= npix
+    # last byte a segment load may touch: clamped to the segment's last byte
+    # (byte path) or an 8-byte load at min(base, end - 8) (only when Pn % 8 == 0)
+    last = 512 * Pn - 1
+    if Pn % 8 == 0:
+        assert Pn >= 8
+    return {"bins": last, "sel": N - 1}
+
+
+def _kb_ref_points(W, H, L, N):
+    out = {"sel": N - 1, "pts": N - 1, "jac_bytes": 28 * N - 1}
+    for lq in range(1, L):
+        out[f"gray_{lq}"] = (W >> lq) * (H >> lq) - 1         # taps clamped into the level image
+    return out
+
+
+def _kb_track(W, H, L, N, maxN, pc):
+    out = {}
+    for layer in range(L):
+        wl, hl = W >> layer, H >> layer
+        # bilinear taps at xdi .. xdi + 1, ydi .. ydi + 1 with 0 <= xdi < wl - 1, 0 <= ydi < hl - 1
+        out[f"img_{layer}"] = (wl - 2) + (hl - 2) * wl + 1 + wl
+        assert out[f"img_{layer}"] <= wl * hl - 1
+    out["depth_tap"] = (W - 1) + (H - 1) * W
+    assert N <= maxN                                     # LDS residuals: maxN entries
+    assert min(N, pc) <= min(maxN, pc)                   # point cache: min(maxN, capacity) entries
+    return out
+
+
+def _check_kernels(f, maxes, max_w, max_h, pcs=(0, 5120, 5632, 8192)):
+    maxW, maxH, maxN = (int(x) for x in maxes)
+    for p in {tuple(int(v) for v in r[:4]) for r in f}:
+        W, H, L, N = p
+        bufsz = {"g8": W * H, "d16": W * H, "bins": W * H, "sel": N, "pts": N}
+        for l in range(1, L):
+            bufsz[f"gray_{l}"] = (W >> l) * (H >> l)
+        pyr = _kb_pyramid(W, H, L, maxW, maxH)
+        for k, v in pyr.items():
+            assert v < bufsz[k], (p, "kb_pyramid", k, v)
+        bins = _kb_bins(W, H, maxW, maxH)
+        assert bins["d16"] < W * H and bins["g8_words"] < W * H and bins["bins"] < W * H
+        assert bins["gray_taps"] < W * H
+        sel = _kb_select(W, H, N)
+        assert sel["bins"] < W * H and sel["sel"] < N
+        ref = _kb_ref_points(W, H, L, N)
+        for k, v in ref.items():
+            if k.startswith("gray_"):
+                assert v < bufsz[k]
+        assert ref["jac_bytes"] < 28 * N
+        for pc in pcs:
+            tr = _kb_track(W, H, L, N, maxN, pc)
+            assert tr["depth_tap"] < W * H
+            for layer in range(1, L):
+                assert tr[f"img_{layer}"] < bufsz[f"gray_{layer}"]
+            assert tr["img_0"] < W * H
+
+
+def _mixed_256():
+    """The mixed-size workload's 256 pairs: sizes drawn by rng(3)."""
+    rng = np.random.default_rng(3)
+    return [SIZES[k] for k in rng.integers(0, 4, 256)]
+
+
+@pytest.mark.parametrize("name,sizes,N", [
+    ("mixed 256 pairs", _mixed_256(), 8192),
+    ("640x480 x 4096", [(640, 480)] * 4096, 8192),
+    ("mixed 8 pairs", SIZES * 2, 2048),
+    ("mixed 6 pairs 12288", [(320, 240), (480, 360), (640, 480)] * 2, 12288),
+])
+def test_batch_launches_stay_in_bounds(name, sizes, N):
+    f, order, maxes = _layout(sizes, N)
+    _check_layout(f, order, maxes, 640, 480, N)
+    _check_kernels(f, maxes, 640, 480)
+
+
+def test_layout_rejects_what_the_kernels_cannot_take():
+    """Sizes the kernels' indexing does not cover are refused at preparation."""
+    for sizes, N in ([[(648, 480)], 2048], [[(100, 120)], 2048], [[(160, 120)], 16384], [[(20, 20)], 256]):
+        f = np.zeros(FIELDS, np.int64)
+        n = len(sizes)
+        arr = (RtPair * n)()
+        cal = np.zeros(96, np.float32)
+        for i, (W, H) in enumerate(sizes):
+            arr[i].ref_gray = arr[i].ref_depth = arr[i].cur_gray = arr[i].cur_depth = 0x1000
+            arr[i].width, arr[i].height = W, H
+            arr[i].calib = cal.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        order = np.zeros(n, np.int32)
+        maxes = np.zeros(3, np.int32)
+        prm = default_params(n_select=N)
+        rc = lib().rt_batch_layout(ctypes.byref(prm), 640, 480, n, arr, f.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
+                                   order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                   maxes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+        assert rc == -1, (sizes, N)

