@@ -273,6 +273,29 @@ int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, void* host, 
 int rt_batch_layout(const rt_params* params, int max_width, int max_height, int n, const rt_pair* pairs,
                     int64_t* fields, int* order, int maxes[3]);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): rt_batch_track without a batch object. The kernels'
+ * working memory (per-pair pyramids, gradient bins, histograms, selections,
+ * point records, descriptor and pose tables, the work queue) is one fixed
+ * device array of RT_NOALLOC_SCRATCH_BYTES (16.9 MiB) that each GPU the library
+ * runs on holds until the process ends; the call allocates nothing. The caller
+ * passes the frames (device pointers), calibration and initial poses as in
+ * rt_pair, and the results; work runs on `stream` (a hipStream_t, NULL for the
+ * null stream) and the call returns when the results are in `out`. Limits:
+ * n <= RT_NOALLOC_MAX_PAIRS, pairs up to RT_NOALLOC_MAX_WIDTH x
+ * RT_NOALLOC_MAX_HEIGHT, params->n_select <= RT_NOALLOC_MAX_POINTS (RT_ERR_ARG
+ * beyond). Every call on one GPU uses that GPU's one array: calls that overlap
+ * on one GPU (other threads, other streams) race. The tracking kernel variant
+ * is fixed (no autotuning); poses agree with rt_batch_track's within rounding. */
+#define RT_NOALLOC_MAX_PAIRS 8
+#define RT_NOALLOC_MAX_WIDTH 640
+#define RT_NOALLOC_MAX_HEIGHT 480
+#define RT_NOALLOC_MAX_POINTS 8192
+#define RT_NOALLOC_SCRATCH_BYTES 17683456
+int rt_batch_track_noalloc(int device, int n, const rt_pair* pairs, const rt_params* params, void* stream,
+                           rt_pair_result* out);
+
 /* ------------------------------------------------------------------ inspection
  * Read an internal device buffer into host memory, for parity tests.
  * what: RT_DBG_*; level: pyramid level where applicable. */

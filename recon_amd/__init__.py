@@ -91,6 +91,7 @@ SIGNATURES = {
     "rt_batch_retrack": (c_int, [c_void_p, P(c_double), P(RtPairResult)]),
     "rt_batch_stream": (c_void_p, [c_void_p]),
     "rt_batch_variant_info": (c_int, [c_int, P(c_int)]),
+    "rt_batch_track_noalloc": (c_int, [c_int, c_int, P(RtPair), P(RtParams), c_void_p, P(RtPairResult)]),
     "rt_batch_layout": (c_int, [P(RtParams), c_int, c_int, c_int, P(RtPair), P(ctypes.c_int64), P(c_int), P(c_int)]),
     "rt_batch_debug_read": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_size_t]),
     "rt_debug_read": (c_int, [c_void_p, c_int, c_int, c_void_p, c_size_t]),

@@ -20,6 +20,7 @@
 #include <string.h>
 #include <algorithm>
 #include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 #include "../../include/rt_tracker.h"
@@ -1123,6 +1124,139 @@ extern "C" int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_resul
             return set_error(RT_ERR_HIP, "pose upload");
     }
     return run_track(b, out);
+}
+
+// ------------------------------------------------------------------ no-alloc batch tracking
+// rt_batch_track for callers that cannot allocate device memory: the pair
+// arenas and tables live in one fixed device array (`scratch`, held by every
+// GPU the library runs on until the process ends) instead of a batch's own
+// allocations; the caller passes frames, calibration and results as for
+// rt_batch_track. Limits: RT_NOALLOC_MAX_PAIRS pairs of at most
+// RT_NOALLOC_MAX_WIDTH x RT_NOALLOC_MAX_HEIGHT and RT_NOALLOC_MAX_POINTS points.
+namespace rt {
+constexpr size_t ca256(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr size_t NA_PAIR_BYTES = 2 * ca256((size_t)RT_NOALLOC_MAX_WIDTH * RT_NOALLOC_MAX_HEIGHT * 4 / 3 + 64) +
+                                 ca256((size_t)RT_NOALLOC_MAX_WIDTH * RT_NOALLOC_MAX_HEIGHT) + ca256(256 * 4) +
+                                 ca256((size_t)RT_NOALLOC_MAX_POINTS * 4) + ca256((size_t)RT_NOALLOC_MAX_POINTS * 16) +
+                                 4 * ca256((size_t)RT_NOALLOC_MAX_POINTS * 28);
+// table region: descriptors, work order, queue, calibration blocks, poses, rms, status
+constexpr size_t NA_DESC = 0, NA_ORDER = 4096, NA_QUEUE = 4352, NA_CALIB = 4608,
+                 NA_TREL_IN = NA_CALIB + ca256((size_t)RT_NOALLOC_MAX_PAIRS * 96 * 4),
+                 NA_TREL_OUT = NA_TREL_IN + ca256((size_t)RT_NOALLOC_MAX_PAIRS * 64),
+                 NA_RMS = NA_TREL_OUT + ca256((size_t)RT_NOALLOC_MAX_PAIRS * 64),
+                 NA_STATUS = NA_RMS + ca256((size_t)RT_NOALLOC_MAX_PAIRS * 4),
+                 NA_TABLES = ca256(NA_STATUS + (size_t)RT_NOALLOC_MAX_PAIRS * 4);
+static_assert(sizeof(PairDesc) * RT_NOALLOC_MAX_PAIRS <= NA_ORDER, "descriptor table");
+static_assert(NA_ORDER + 4 * RT_NOALLOC_MAX_PAIRS <= NA_QUEUE, "order table");
+constexpr size_t NA_SCRATCH_BYTES = NA_TABLES + (size_t)RT_NOALLOC_MAX_PAIRS * NA_PAIR_BYTES;
+static_assert(NA_SCRATCH_BYTES == RT_NOALLOC_SCRATCH_BYTES, "the header states the scratch size");
+}  // namespace rt
+
+// Host code never uses this name as an address: the per-device address comes
+// from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) unsigned char scratch[rt::NA_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_na_mutex;
+void* g_na_ptr[64] = {};
+// The device address of `scratch` on the current device, looked up once per
+// device (with that device current).
+int noalloc_scratch(int device, char** out) {
+    std::lock_guard<std::mutex> lk(g_na_mutex);
+    if (!g_na_ptr[device]) {
+        void* ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) {
+            (void)hipGetLastError();
+            return set_error(RT_ERR_HIP, "rt_batch_track_noalloc: hipGetSymbolAddress of the scratch array failed");
+        }
+        g_na_ptr[device] = ptr;
+    }
+    *out = (char*)g_na_ptr[device];
+    return RT_OK;
+}
+// The tracking kernel variant of the no-alloc path (no autotuning): 1024
+// threads with residuals and the point cache in LDS where it covers the
+// points, else 1024 threads with residuals in registers.
+int noalloc_variant(int maxN) {
+    int fallback = -1;
+    for (int v = 0; v < NUM_VARIANTS; v++) {
+        const rt::Variant& k = rt::kVariants[v];
+        if (k.bs != 1024) continue;
+        if (k.rz && k.pc && k.ppt == 4 && lds_variant_ready(k, maxN)) return v;
+        if (!k.rz && !k.pc && k.bs * k.ppt >= maxN && fallback < 0) fallback = v;
+    }
+    return fallback;
+}
+}  // namespace
+
+extern "C" int rt_batch_track_noalloc(int device, int n, const rt_pair* pairs, const rt_params* params, void* stream,
+                                      rt_pair_result* out) {
+    if (!pairs || !out || n < 1) return set_error(RT_ERR_ARG, "rt_batch_track_noalloc: bad argument");
+    if (n > RT_NOALLOC_MAX_PAIRS)
+        return set_error(RT_ERR_ARG, "rt_batch_track_noalloc: more pairs than RT_NOALLOC_MAX_PAIRS");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev || device >= 64)
+        return set_error(RT_ERR_ARG, "rt_batch_track_noalloc: no such device");
+    if (hipSetDevice(device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    rt_params prm;
+    if (params) prm = *params; else rt_default_params(&prm);
+    if (prm.n_select < 256 || prm.n_select % 256 || prm.n_select > RT_NOALLOC_MAX_POINTS)
+        return set_error(RT_ERR_ARG, "rt_batch_track_noalloc: n_select must be a multiple of 256 up to RT_NOALLOC_MAX_POINTS");
+    if (prm.mad_bins != 64 && prm.mad_bins != 256) return set_error(RT_ERR_ARG, "mad_bins must be 64 or 256");
+    for (int i = 0; i < n; i++)
+        if (pairs[i].width > RT_NOALLOC_MAX_WIDTH || pairs[i].height > RT_NOALLOC_MAX_HEIGHT)
+            return set_error(RT_ERR_ARG, "rt_batch_track_noalloc: pair larger than RT_NOALLOC_MAX_WIDTH x RT_NOALLOC_MAX_HEIGHT");
+    if (pair_arena_bytes(RT_NOALLOC_MAX_WIDTH, RT_NOALLOC_MAX_HEIGHT, RT_NOALLOC_MAX_POINTS) != NA_PAIR_BYTES)
+        return set_error(RT_ERR_STATE, "rt_batch_track_noalloc: arena layout mismatch");
+    char* base = nullptr;
+    int rc = noalloc_scratch(device, &base);
+    if (rc) return rc;
+    const BatchMem m{base + NA_TABLES, NA_PAIR_BYTES, RT_NOALLOC_MAX_WIDTH, RT_NOALLOC_MAX_HEIGHT,
+                     (size_t)RT_NOALLOC_MAX_POINTS, (float*)(base + NA_CALIB), (float*)(base + NA_TREL_IN),
+                     (float*)(base + NA_TREL_OUT), (float*)(base + NA_RMS), (int*)(base + NA_STATUS)};
+    // host tables in this frame; the stream is synchronised before return, so
+    // the queued copies have read them by then
+    PairDesc desc[RT_NOALLOC_MAX_PAIRS];
+    float calib[RT_NOALLOC_MAX_PAIRS * 96], trel[RT_NOALLOC_MAX_PAIRS * 16];
+    int order[RT_NOALLOC_MAX_PAIRS];
+    int maxW = 0, maxH = 0, maxN = 0;
+    if ((rc = build_batch(prm, m, n, pairs, desc, calib, trel, order, maxW, maxH, maxN))) return rc;
+    const int variant = noalloc_variant(maxN);
+    if (variant < 0) return set_error(RT_ERR_UNSUPPORTED, "rt_batch_track_noalloc: no kernel variant covers the point count");
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+    hipStream_t s = (hipStream_t)stream;
+    if (hipMemcpyAsync(base + NA_DESC, desc, sizeof(PairDesc) * n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + NA_ORDER, order, 4 * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + NA_CALIB, calib, 96 * 4 * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess ||
+        hipMemcpyAsync(base + NA_TREL_IN, trel, 64 * (size_t)n, hipMemcpyHostToDevice, s) != hipSuccess) {
+        (void)hipStreamSynchronize(s);
+        return set_error(RT_ERR_HIP, "rt_batch_track_noalloc: table upload");
+    }
+    const BatchView v{(const PairDesc*)(base + NA_DESC), (const int*)(base + NA_ORDER), (int*)(base + NA_QUEUE),
+                      n, maxW, maxH, maxN, prm.solver, prm.robust, prm.mad_bins};
+    (void)hipGetLastError();
+    launch_prep(v, s);
+    launch_track(v, variant, std::min(n, cus), s);
+    hipError_t le = hipGetLastError();
+    float tr[RT_NOALLOC_MAX_PAIRS * 16], rms[RT_NOALLOC_MAX_PAIRS];
+    int st[RT_NOALLOC_MAX_PAIRS];
+    hipError_t ce = hipMemcpyAsync(tr, m.trel_out, 64 * (size_t)n, hipMemcpyDeviceToHost, s);
+    if (ce == hipSuccess) ce = hipMemcpyAsync(rms, m.rms, 4 * (size_t)n, hipMemcpyDeviceToHost, s);
+    if (ce == hipSuccess) ce = hipMemcpyAsync(st, m.status, 4 * (size_t)n, hipMemcpyDeviceToHost, s);
+    const hipError_t se = hipStreamSynchronize(s);
+    if (le != hipSuccess) return set_error(RT_ERR_HIP, "rt_batch_track_noalloc: launch failed");
+    if (ce != hipSuccess || se != hipSuccess) return set_error(RT_ERR_HIP, "rt_batch_track_noalloc: result copy");
+    for (int i = 0; i < n; i++) {
+        double Td[16], Ti[16];
+        for (int k = 0; k < 16; k++) Td[k] = tr[16 * i + k];
+        invert_rt4(Td, Ti);
+        memcpy(out[i].Trel, Td, sizeof(Td));
+        memcpy(out[i].T, Ti, sizeof(Ti));
+        out[i].rms = rms[i];
+        out[i].status = st[i];
+    }
+    return RT_OK;
 }
 
 // ------------------------------------------------------------------ layout inspection
