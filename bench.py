@@ -181,11 +181,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=("learned", "small", "single", "sequence"), default="learned",
+    ap.add_argument("--workload", choices=("learned", "small", "single", "sequence", "mixed"), default="learned",
                     help="learned: 4096 pairs at 640x480, large motions, network proposals + batched tracking "
                          "(the headline); small: the same batch from identity at small motions; single: one pair "
                          "at 320x240 on the single-pair tracker; sequence: 300 frames at 640x480, bi-objective, "
-                         "keyframes every 10 frames fused with the next 5")
+                         "keyframes every 10 frames fused with the next 5; mixed: 256 pairs of mixed sizes in one "
+                         "persistent launch")
     ap.add_argument("--pairs", type=int, default=4096)
     ap.add_argument("--plain-pairs", type=int, default=64)
     ap.add_argument("--width", type=int, default=640)
@@ -199,8 +200,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU threads (0: nproc)")
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--extras", action="store_true",
-                    help="also run the single-pair and sequence cases (the mixed-size batch leg is disabled: it "
-                         "faulted the GPU once and is kept off the GPU until its cause is shown)")
+                    help="also run the single-pair and sequence cases of the earlier build's extra leg "
+                         "(--workload single / sequence give them their own lines)")
     ap.add_argument("--track-only", action="store_true",
                     help="prepare the batch and run --steps tracking launches only (for counter collection)")
     ap.add_argument("--tsdf", action="store_true",
@@ -226,6 +227,9 @@ def main():
         return
     if args.workload == "sequence":
         sequence_bench(args)
+        return
+    if args.workload == "mixed":
+        mixed_bench(args)
         return
 
     from recon_amd import PHOTOMETRIC
@@ -789,85 +793,104 @@ def extra_sequence():
             "max_mm": float(e[:, 1].max())}
 
 
-def extra_mixed_batch(threads):
-    """256 pairs of mixed sizes (160x120, 320x240, 480x360, 640x480; motion <= 3
-    deg / 50 mm) in one persistent launch, 8192 points each; frames/s of the
-    tracking step and per-pair error against the FP64 oracle on a sample.
-
-    Disabled: an earlier build faulted the GPU in this leg, and the case stays
-    off the GPU until the cause is shown and fixed."""
-    raise RuntimeError("the mixed-size batch leg is disabled: it faulted the GPU once and its cause is not yet shown")
-    import threading
-
+def mixed_bench(args):
+    """The mixed-size batch: 256 pairs with sizes drawn uniformly from 160x120,
+    320x240, 480x360 and 640x480, motions <= 3 deg / 50 mm, 8192 points each,
+    4 levels at 640 (else 3), iterations [1, 3, 10, 5], from identity, in one
+    persistent launch with the autotuner's variant. A step = keyframe
+    preparation + tracking of the batch from frames resident in HBM to poses
+    on the host. Roofline of the tracking launch: 60 B per point-iteration
+    summed over the pairs (HIP events on the batch's stream). Error per pair
+    against the FP64 oracle tracker from the same start, on a sample."""
     import oracle
     from recon_amd import PHOTOMETRIC, synth
-    from recon_amd.hip import DeviceBuffer
+    from recon_amd.hip import Event, stream_sync
     from recon_amd.tracker import BatchTracker
-    sizes = [(160, 120), (320, 240), (480, 360), (640, 480)]
-    rng = np.random.default_rng(3)
-    n = 256
-    pick = rng.integers(0, 4, n)
-    pairs, host, keep, gts = [], [], [], []
-    for i in range(n):
-        W, H = sizes[pick[i]]
-        g0, d0, g1, d1, Tgt = synth.make_pair(3000 + i, W, H, 3.0, 50.0) if i < 16 else (None,) * 5
-        if g0 is None:
-            # the rest rendered on the device
-            from recon_amd import synth_lib
-            S = synth_lib()
-            M = synth.sample_motion(7000 + i, 3.0, 50.0)
-            bufs = [DeviceBuffer(W * H), DeviceBuffer(W * H * 2), DeviceBuffer(W * H), DeviceBuffer(W * H * 2)]
-            sc = S.rs_scene_create(ctypes.c_uint64(3000 + i))
-            fx, fy, cx, cy = synth.pinhole(W, H)
-            dp = ctypes.POINTER(ctypes.c_double)
-            for f, T in ((0, np.eye(4)), (1, M)):
-                Tf = np.ascontiguousarray(T, np.float64).ravel()
-                S.rs_render_device(sc, Tf.ctypes.data_as(dp), W, H, fx, fy, cx, cy, ctypes.c_uint64(9000 + 2 * i + f),
-                                   0.01, bufs[2 * f].ptr, bufs[2 * f + 1].ptr, None)
-            S.rs_scene_destroy(sc)
-            Tgt = np.linalg.inv(M)
-        else:
-            bufs = [DeviceBuffer.from_array(a) for a in (g0, d0, g1, d1)]
-            host.append((i, W, H, g0, d0, g1, d1))
-        keep.append(bufs)
-        gts.append(Tgt)
-        pairs.append(dict(ref_gray=bufs[0].ptr, ref_depth=bufs[1].ptr, cur_gray=bufs[2].ptr, cur_depth=bufs[3].ptr,
-                          width=W, height=H, calib=synth.calib_block_np(W, H)))
-    # the frames were rendered on the null stream; the batch runs on a
-    # non-blocking stream that does not wait for it
-    from recon_amd.hip import hip
-    hip().hipDeviceSynchronize()
-    bt = BatchTracker(n, 640, 480, mode=PHOTOMETRIC, n_select=8192, iters=(1, 3, 10, 5))
+    n, N, iters = 256, 8192, (1, 3, 10, 5)
+    t0 = time.time()
+    pairs, g, d = synth.render_mixed_batch(n, 3, 3.0, 50.0)
+    mix = {f"{W}x{H}": sum(1 for p in pairs if (p["width"], p["height"]) == (W, H)) for W, H in synth.MIXED_SIZES}
+    log(f"inputs: {n} mixed pairs {mix} rendered in {time.time() - t0:.1f}s")
+    bt = BatchTracker(n, 640, 480, mode=PHOTOMETRIC, n_select=N, iters=iters)
     prep = bt.prepare(pairs)
-    bt.track(prep)
-    reps = 5
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        Trel, _, _, st = bt.track(prep)
-    dt = (time.perf_counter() - t0) / reps
-    res = {}
+    stream = bt.L.rt_batch_stream(bt.h)
+    for _ in range(max(1, args.warmup)):
+        Trel, _, rms, st = bt.track(prep)
+    steps = max(1, args.steps)
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        Trel, _, rms, st = bt.track(prep)
+    step_s = (time.perf_counter() - t1) / steps
+    e0, e1 = Event(), Event()
+    e0.record(stream)
+    for _ in range(3):
+        bt.L.rt_batch_retrack(bt.h, None, None)
+    e1.record(stream)
+    e1.synchronize()
+    stream_sync(stream)
+    k_ms = e0.elapsed_ms(e1) / 3
+    alg = sum(sum(iters[:(4 if p["width"] == 640 else 3)]) * N * BYTES_PER_POINT_ITER for p in pairs)
+    achieved = alg / (k_ms * 1e-3) / 1e9
+    # FP64 oracle from identity on a sample: 8 pairs of each size
+    pick = []
+    for W, H in synth.MIXED_SIZES:
+        pick += [i for i, p in enumerate(pairs) if (p["width"], p["height"]) == (W, H)][:8]
+    total = sum(p["width"] * p["height"] for p in pairs)
+    gh = g.to_array(np.uint8, 2 * total)
+    dh = d.to_array(np.uint16, 2 * total)
 
-    def work(item):
-        i, W, H, g0, d0, g1, d1 = item
-        L = 4 if W == 640 else 3
-        o = oracle.Tracker(W, H, synth.calib_block_np(W, H), mode=oracle.PHOTOMETRIC, levels=L, iters=(1, 3, 10, 5),
-                           nselect=8192, dtype=np.float64)
+    def frames(p):
+        W, H = p["width"], p["height"]
+        o0, o1 = p["off"]
+        return (gh[o0:o0 + W * H].reshape(H, W), dh[o0:o0 + W * H].reshape(H, W),
+                gh[o1:o1 + W * H].reshape(H, W), dh[o1:o1 + W * H].reshape(H, W))
+
+    def track_oracle(i, dtype):
+        p = pairs[i]
+        W, H = p["width"], p["height"]
+        o = oracle.Tracker(W, H, p["calib"], mode=oracle.PHOTOMETRIC, levels=4 if W == 640 else 3, iters=iters,
+                           nselect=N, dtype=dtype)
+        g0, d0, g1, d1 = frames(p)
         o.set_reference(g0, d0); o.set_current(g1, d1); o.track()
-        res[i] = o.poses()[0]
+        return o.poses()[0]
 
-    ts = [threading.Thread(target=work, args=(h,)) for h in host[:max(1, min(threads, 16))]]
-    for t in ts:
-        t.start()
-    for t in ts:
-        t.join()
-    from recon_amd.synth import pose_error
-    e = np.array([pose_error(Trel[i], res[i]) for i in res])
-    et = np.array([pose_error(Trel[i], gts[i]) for i in range(n)])
-    return {"pairs": n, "sizes": {f"{w}x{h}": int((pick == k).sum()) for k, (w, h) in enumerate(sizes)},
-            "fps": round(n / dt, 1), "ms_per_step": round(dt * 1e3, 3), "status_ok": int((st == 0).sum()),
-            "vs_fp64_oracle_max_deg": float(e[:, 0].max()), "vs_fp64_oracle_max_mm": float(e[:, 1].max()),
-            "oracle_sample": len(res), "vs_truth_median_deg": float(np.median(et[:, 0])),
-            "vs_truth_median_mm": float(np.median(et[:, 1]))}
+    from concurrent.futures import ThreadPoolExecutor
+    model, nproc = host_cpu()
+    with ThreadPoolExecutor(max(1, min(16, nproc))) as ex:
+        ref64 = list(ex.map(lambda i: track_oracle(i, np.float64), pick))
+    per_pair = []
+    for i, R in zip(pick, ref64):
+        r, t = synth.pose_error(Trel[i], R)
+        rt, tt = synth.pose_error(Trel[i], pairs[i]["gt"])
+        per_pair.append({"pair": i, "size": f"{pairs[i]['width']}x{pairs[i]['height']}", "vs_fp64_deg": round(r, 6),
+                         "vs_fp64_mm": round(t, 5), "vs_truth_deg": round(rt, 4), "vs_truth_mm": round(tt, 3)})
+    ok = [e for e in per_pair if e["vs_truth_deg"] <= 0.1 and e["vs_truth_mm"] <= 2.0]
+    cpu = None
+    if args.cpu_seconds > 0:
+        cycle = [0]
+
+        def work():
+            cycle[0] += 1
+            track_oracle(pick[cycle[0] % len(pick)], np.float32)
+            return 1
+        cpu = cpu_sample(work, args.cpu_seconds, args.cpu_threads,
+                         f"the {len(pick)} sampled pairs (8 of each size) cycled, FP32 oracle tracker (the reference's "
+                         "kernels restated on the CPU), one pair per thread")
+    log(f"mixed batch: {n / step_s:.0f} frames/s ({step_s * 1e3:.2f} ms per step); kb_track {k_ms:.3f} ms, "
+        f"{achieved:.0f} GB/s algorithmic; status ok {int((st == 0).sum())}/{n}; sample vs FP64 oracle: max "
+        f"{max(e['vs_fp64_deg'] for e in ok) if ok else None} deg over {len(ok)} pairs near the truth")
+    _line("tracked frames/s of 256 mixed-size pairs (160x120 .. 640x480) in one persistent launch", n / step_s,
+          "frames/s", args, steps, step_s, "f32+f64",
+          {"workload": "256 pairs, sizes uniform over 160x120, 320x240, 480x360, 640x480 (numpy default_rng(3)), "
+                       "rotation <= 3 deg, translation <= 50 mm, 8192 points, iterations [1, 3, 10, 5] (3 levels below "
+                       "640 wide), identity start; frames resident in HBM", "pairs": n, "sizes": mix, "points": N},
+          status_ok=int((st == 0).sum()),
+          pose_error={"sample": len(pick), "near_truth": len(ok), "per_pair": per_pair},
+          roofline={"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "algorithmic_bytes": alg,
+                    "kernel": "kb_track (persistent Gauss-Newton), HIP events on the batch's stream",
+                    "kernel_ms": round(k_ms, 4)},
+          cpu_baseline=cpu)
 
 
 def fp64_reference(g, d, gt, n, W, H, L, N, iters, cal, threads):

@@ -216,8 +216,12 @@ int rt_track_sequence(rt_tracker* t, const uint8_t* gray_dev, const uint16_t* de
 /* ------------------------------------------------------------------ batches
  * Many independent frame pairs of mixed sizes tracked by one persistent
  * launch: each workgroup owns one pair at a time and runs its whole
- * coarse-to-fine Gauss-Newton loop (photometric mode), taking pairs from a
- * device work queue. No reference counterpart: the reference tracks one pair. */
+ * coarse-to-fine Gauss-Newton loop, taking pairs from a device work queue, in
+ * the mode of the batch's parameters: photometric, geometric (point-to-plane
+ * ICP with Student-t weights) or bi-objective (both systems fused by
+ * linearFuse with params->lambda), as ShapeTracker / PhotoTracker
+ * (shapetracker.cpp:357-394, phototracker.cpp:666-689) do for one pair. No
+ * reference counterpart for the batching: the reference tracks one pair. */
 typedef struct {
     const uint8_t* ref_gray;    /* device pointers, width*height each */
     const uint16_t* ref_depth;
@@ -254,22 +258,28 @@ void* rt_batch_stream(rt_batch* b);
  * environment variable RT_BATCH_VARIANT=v pins variant v): info receives
  * {workgroup size, points per thread (unroll depth where residuals live in
  * LDS), residuals in LDS (0/1), LDS point-cache capacity in points, waves per
- * SIMD}. Returns RT_ERR_ARG past the end of the table. */
-int rt_batch_variant_info(int v, int info[5]);
+ * SIMD, tracking mode}. Returns RT_ERR_ARG past the end of the table. */
+int rt_batch_variant_info(int v, int info[6]);
 /* Inspection of one pair of the last prepared batch (for parity tests):
  * RT_DBG_GRAD_HIST, RT_DBG_SELECTED, RT_DBG_POINTS, RT_DBG_JACOBIAN (level),
- * RT_DBG_TREL (the pair's initial pose); layouts as for single trackers. */
+ * RT_DBG_TREL (the pair's initial pose); in the geometric and bi-objective
+ * modes also RT_DBG_REF_CLOUD, RT_DBG_CUR_CLOUD (level) and, after tracking,
+ * RT_DBG_ICP_COUNT and RT_DBG_ICP_VAR (level) of the last tracking launch;
+ * layouts as for single trackers. */
 int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, void* host, size_t bytes);
 
 /* Inspection for bounds checks on the host (no device is touched): the pair
  * descriptors rt_batch_prepare builds for n pairs in a batch of max_width x
- * max_height, as byte offsets from the batch's arena. fields: 24 int64 per
+ * max_height, as byte offsets from the batch's arena. fields: 40 int64 per
  * pair: W, H, levels, points, simple-calibration flag, offsets of the gray
  * pyramid levels 1-3 of the keyframe and of the current frame (-1 where the
  * level is absent), of the gradient bins, gradient histogram, selection,
  * points and the point records of levels 0-3, the pair's arena slot start
- * and the slot size, then 3 zeros. order: the work order (n ints); maxes:
- * the largest width, height and point count. */
+ * and the slot size, then in the geometric modes (else -1) the offsets of the
+ * keyframe cloud, its normals and the current cloud of levels 0-3, of the ICP
+ * selection, its squared residuals, the ICP counts and variances, then 3
+ * zeros. order: the work order (n ints); maxes: the largest width, height
+ * and point count. */
 int rt_batch_layout(const rt_params* params, int max_width, int max_height, int n, const rt_pair* pairs,
                     int64_t* fields, int* order, int maxes[3]);
 

@@ -115,6 +115,11 @@ SIGNATURES = {
     "rt_cnn_target_to_pose": (c_int, [P(RtCnnConfig), c_int, P(c_float), P(c_double)]),
     "rt_batch_prepare": (c_int, [c_void_p, c_int, P(RtPair)]),
     "rt_batch_propose": (c_int, [c_void_p, c_void_p]),
+    # include/rt_sequence.h
+    "rt_sequence_default_params": (None, [c_void_p]),
+    "rt_sequence_create": (c_int, [c_int, c_int, c_int, P(c_float), c_void_p, c_void_p, P(c_void_p)]),
+    "rt_sequence_destroy": (None, [c_void_p]),
+    "rt_sequence_run": (c_int, [c_void_p, c_void_p, c_void_p, c_int, P(c_double), P(c_int), P(c_float), P(c_int)]),
     # include/rt_sensor.h
     "rt_calib_load_xml": (c_int, [ctypes.c_char_p, c_int, c_int, P(c_float), P(c_int)]),
     "rt_calib_parse_xml": (c_int, [ctypes.c_char_p, c_size_t, c_int, c_int, P(c_float), P(c_int)]),

@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 ROOT = os.path.dirname(HERE)
 
-HIP_SOURCES = ["rt_kernels.hip", "rt_tracker.hip", "rt_batch.hip", "cnn.hip", "synth.hip", "rt_sensor.hip", "rt_trajectory.hip", "rt_tsdf.hip"]
+HIP_SOURCES = ["rt_kernels.hip", "rt_tracker.hip", "rt_batch.hip", "cnn.hip", "synth.hip", "rt_sensor.hip", "rt_trajectory.hip", "rt_tsdf.hip", "rt_sequence.hip"]
 HIPCC_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", "-shared", "--offload-arch=gfx950",
     # every FP32 value is the rounded result of the written operation sequence

@@ -50,6 +50,17 @@ struct PairDesc {
     float* trel_out;        // 16
     float* rms;             // 1
     int* status;            // 1
+    // geometric and bi-objective modes: keyframe cloud (x, y, z, mask) and
+    // normals, current cloud (x, y, z, valid) per level; the compacted ICP
+    // points and squared residuals of the level being tracked; per level the
+    // ICP point count and the three Student-t variance sums
+    float4* rpos[4];
+    float4* rnrm[4];
+    float4* cpos[4];
+    int* isel;
+    float* ir2;
+    int* icnt;              // 4
+    float* ivar;            // 4 x 3
 };
 
 // ------------------------------------------------------------------ batched preparation
@@ -428,8 +439,170 @@ __global__ __launch_bounds__(256) void kb_ref_points(const PairDesc* __restrict_
     ref_point(i, U16DepthSrc{d.d16[0]}, U8Src{d.g8[0], lut}, d.gray[0][1], d.gray[0][2], d.gray[0][3], d.W, d.H, d.L,
               d.N, cl, 1e-3f, d.sel, d.pts, d.jac[0], d.jac[1], d.jac[2], d.jac[3]);
 }
+// Geometric modes: the keyframe cloud with normals and the current cloud at
+// level 0 from the 16-bit depths (ShapeTracker::setReferencePoints /
+// generatePyramids, shapetracker.cpp:195-211,254-258), then the masked 2x2
+// pyramids level by level. The per-pixel bodies are the single-pair kernels'
+// (ref_cloud_pixel, cur_cloud_pixel, cloud_down_pixel), reading the depth
+// through the same exact conversion, so the clouds are the same bit for bit.
+__global__ __launch_bounds__(256) void kb_clouds(const PairDesc* __restrict__ P) {
+    const PairDesc& d = P[blockIdx.z];
+    const int x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 16 + (threadIdx.x >> 4);
+    if (x >= d.W || y >= d.H) return;
+    ref_cloud_pixel(U16DepthSrc{d.d16[0]}, d.W, d.H, x, y, d.calib, d.rpos[0], d.rnrm[0]);
+    cur_cloud_pixel(U16DepthSrc{d.d16[1]}, d.W, x + y * d.W, d.calib, d.cpos[0]);
+}
+__global__ __launch_bounds__(256) void kb_clouds_down(const PairDesc* __restrict__ P, int level) {
+    const PairDesc& d = P[blockIdx.y];
+    if (level >= d.L) return;
+    const int Wd = d.W >> level, Hd = d.H >> level;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= Wd * Hd) return;
+    cloud_down_pixel(d.rpos[level - 1], d.rnrm[level - 1], Wd, i, d.rpos[level], d.rnrm[level]);
+    cloud_down_pixel(d.cpos[level - 1], nullptr, Wd, i, d.cpos[level], nullptr);
+}
+
 __global__ void kb_reset_queue(int* q) {
     if (threadIdx.x == 0) q[0] = 0;
+}
+
+// ------------------------------------------------------------------ geometric terms in one workgroup
+// ICP selection of one level (icpResidualKernel2 + selectValidIndex +
+// packIndex, rgb2gray.cu:223-260,827-850): ordered compaction of the valid
+// keyframe pixels into d.isel / d.ir2 (squared residual), BS pixels a step:
+// a ballot per wave, the waves' counts in LDS. Ascending pixel order, as the
+// single-pair compaction. Returns the count (the same in every thread).
+template <int BS>
+__device__ int icp_select_wg(const PairDesc& d, int layer, const float* T, const float* c) {
+    constexpr int NW = BS / 64;
+    __shared__ int wc[NW];
+    const int t = threadIdx.x, wave = t >> 6, lane = lane_id();
+    const int w = d.W >> layer, h = d.H >> layer, npix = w * h;
+    const float a = 1.0f / (float)(1 << layer), b = 0.5f * (a - 1.0f);
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+    RT_GLOBAL int* isel = gmem(d.isel);
+    RT_GLOBAL float* ir2 = gmem(d.ir2);
+    int base = 0;
+    for (int o0 = 0; o0 < npix; o0 += BS) {
+        const int o = o0 + t;
+        float r = 0.0f;
+        const bool valid = o < npix && icp_mask_pixel(d.rpos[layer], d.rnrm[layer], d.cpos[layer], w, h, o, T, c, a, b, r);
+        const uint64_t bl = __ballot(valid);
+        if (lane == 0) wc[wave] = __popcll(bl);
+        __syncthreads();
+        int off = 0, tot = 0;
+#pragma unroll
+        for (int q = 0; q < NW; q++) {
+            const int cq = wc[q];
+            off += q < wave ? cq : 0;
+            tot += cq;
+        }
+        if (valid) {
+            const int k = base + off + __popcll(bl & lt);
+            isel[k] = o;
+            ir2[k] = r * r;
+        }
+        base += tot;
+        __syncthreads();
+    }
+    return base;
+}
+// generateStudentTWeights (rgb2gray.cu:470-568, 854-877) in one workgroup: the
+// three variance rounds with the single-pair kernels' arithmetic and trees
+// (the product truncated to int per point, a 1024-wide FP32 tree per block of
+// 1024 points, the block sums truncated to int and reduced by the same tree,
+// clamped to at least n), so the sums are the same bit for bit. var: three
+// values in LDS, unchanged when n = 0 (no solve follows then).
+template <int BS>
+__device__ void student_t_wg(const float* __restrict__ r2, int n, float* var) {
+    __shared__ float p[1024];
+    __shared__ float bsum[512];  // n <= W*H <= 512 * 1024 (checked at preparation)
+    const int t = threadIdx.x;
+    if (n <= 0) return;
+    const int nb = (n + 1023) / 1024;
+    auto tree = [&]() {
+        for (int s2 = 512; s2 >= 1; s2 >>= 1) {
+            for (int e = t; e < s2; e += BS) p[e] += p[e + s2];
+            __syncthreads();
+        }
+    };
+    for (int round = 1; round <= 3; round++) {
+        const float prev = round > 1 ? var[round - 2] : 0.0f;
+        for (int bk = 0; bk < nb; bk++) {
+            for (int e = t; e < 1024; e += BS) {
+                const int i = bk * 1024 + e;
+                int sum = 0;
+                if (i < n) {
+                    const float x = r2[i];
+                    const float w = round == 1 ? 6 / (5 + x / 100.0f) : 6 / (5 + x / (prev / (float)n));
+                    sum += x * w;
+                }
+                p[e] = (float)sum;
+            }
+            __syncthreads();
+            tree();
+            if (t == 0) bsum[bk] = p[0];
+            __syncthreads();
+        }
+        if (nb == 1) {
+            if (t == 0) var[round - 1] = bsum[0];
+        } else {
+            for (int e = t; e < 1024; e += BS) {
+                int ss = 0;
+                for (int q = e; q < nb; q += 1024) ss += bsum[q];
+                p[e] = (float)ss;
+            }
+            __syncthreads();
+            tree();
+            if (t == 0) {
+                float v = p[0];
+                const float mn = (float)n;
+                if (v < mn) v = mn;
+                var[round - 1] = v;
+            }
+        }
+        __syncthreads();
+    }
+}
+// The geometric system of one iteration (icpResidualIndexKernel2,
+// rgb2gray.cu:323-390, and the 27 dot products): per thread in FP32 over its
+// points, per wave by halving exchanges, across waves in FP64 into sysg[27].
+template <int BS>
+__device__ void icp_system_wg(const PairDesc& d, int layer, int n, float v3, const float* T, const float* c,
+                              double (*wsum)[29], double* sysg) {
+    constexpr int NW = BS / 64;
+    const int t = threadIdx.x, wave = t >> 6, lane = lane_id();
+    const int w = d.W >> layer, h = d.H >> layer;
+    const float a = 1.0f / (float)(1 << layer), b = 0.5f * (a - 1.0f);
+    const RT_GLOBAL int* isel = gmem(d.isel);
+    const RT_GLOBAL float* ir2 = gmem(d.ir2);
+    float acc[27];
+#pragma unroll
+    for (int k = 0; k < 27; k++) acc[k] = 0.0f;
+    for (int i = t; i < n; i += BS) {
+        float res, J[6];
+        icp_point_terms(d.rpos[layer], d.rnrm[layer], d.cpos[layer], w, h, isel[i], ir2[i], v3, n, T, c, a, b, 1e-3f,
+                        res, J);
+        int q = 0;
+#pragma unroll
+        for (int r0 = 0; r0 < 6; r0++)
+#pragma unroll
+            for (int c0 = r0; c0 < 6; c0++) acc[q++] += J[r0] * J[c0];
+#pragma unroll
+        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += J[r0] * res;
+    }
+    float v32[32];
+#pragma unroll
+    for (int k = 0; k < 32; k++) v32[k] = k < 27 ? acc[k] : 0.0f;
+    const float tot = wave_sum32(v32);
+    if ((lane & 1) == 0 && (lane >> 1) < 27) wsum[wave][lane >> 1] = (double)tot;
+    __syncthreads();
+    if (t < 27) {
+        double v = 0.0;
+        for (int w2 = 0; w2 < NW; w2++) v += wsum[w2][t];
+        sysg[t] = v;
+    }
+    __syncthreads();
 }
 
 // ------------------------------------------------------------------ persistent tracking
@@ -438,6 +611,7 @@ struct TrackCfg {
     int robust;
     int mad_bins;
     float scale_out;
+    float lambda;  // bi-objective weight of the geometric system (linearFuse)
     int pc_cap;  // point-cache capacity (points; PC variants)
     int rz_len;  // length of the RZ residual array (the batch's largest point count)
 };
@@ -459,15 +633,24 @@ constexpr int PC_MAX = 8192;
 // would overlap one pair's serial phases with the other's point passes, were
 // measured slower than one 1024-thread workgroup at 640x480: the kernel is
 // bound by memory traffic, not by those phases.
-template <int BS, int PPT, int RZ, int WPE = 1, int PC = 0>
+// MODE: the tracking mode (RT_GEOMETRIC 0, RT_PHOTOMETRIC 1, RT_BIOBJECTIVE 2).
+// The geometric modes add, per level, the ICP selection and Student-t weights
+// at the level's first iteration and the geometric system every iteration
+// (ShapeTracker::updateLinearSystem, shapetracker.cpp:357-382), fused with the
+// photometric one by linearFuse in the bi-objective mode (:392-394).
+template <int BS, int PPT, int RZ, int WPE = 1, int PC = 0, int MODE = 1>
 __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__ P, const int* __restrict__ order,
                                                     int npairs, int* queue, TrackCfg cfg) {
     constexpr int NW = BS / 64;
+    // system terms: photometric 27 | geometric 27 (geometric modes) | the two
+    // sums of the photometric residual RMS
+    constexpr int NSYS = MODE == 1 ? 29 : 56, RMS0 = MODE == 1 ? 27 : 54;
     __shared__ float cal[96];
     __shared__ float T[16];
     __shared__ unsigned hist[256];
     __shared__ double wsum[NW][29];
-    __shared__ double sys[29];
+    __shared__ double sys[NSYS];
+    __shared__ float s_var[3];
     __shared__ double ws[SOLVE_WS];
     __shared__ int s_pair, s_med;
     __shared__ int s_ok;
@@ -508,12 +691,14 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
             const JacView jv = jac_view(d.jac[layer], N);
             const int wl = W >> layer, hl = H >> layer;
             const float a = 1.0f / (float)(1 << layer), b = 0.5f * (a - 1.0f);
+            int nicp = 0;  // ICP points of the level (geometric modes)
             for (int it = 0; it < nit; it++) {
                 for (int k = t; k < 256; k += BS) hist[k] = 0u;
                 float Tl[16];
 #pragma unroll
                 for (int k = 0; k < 16; k++) Tl[k] = uni(T[k]);
                 __syncthreads();
+              if constexpr (MODE != 0) {
                 // residual pass: r and zWeight stay in registers (RZ 0) or go to LDS (RZ 1)
                 float r[RZ ? 1 : PPT], zw[RZ ? 1 : PPT];
                 auto residual = [&](int i, float& rr, float& zz) {
@@ -634,18 +819,34 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
                 if (t < 29) {
                     double v = 0.0;
                     for (int w2 = 0; w2 < NW; w2++) v += wsum[w2][t];
-                    sys[t] = v;
+                    sys[t < 27 ? t : RMS0 + t - 27] = v;
                 }
                 __syncthreads();
+              }
+                if constexpr (MODE != 1) {
+                    // the level's ICP points and Student-t weights at its first
+                    // iteration (under the pose the iteration starts from)
+                    if (it == 0) {
+                        nicp = icp_select_wg<BS>(d, layer, Tl, cl);
+                        student_t_wg<BS>(d.ir2, nicp, s_var);
+                        if (t == 0) {
+                            d.icnt[layer] = nicp;
+                            for (int k = 0; k < 3; k++) d.ivar[layer * 3 + k] = nicp > 0 ? s_var[k] : 0.0f;
+                        }
+                    }
+                    icp_system_wg<BS>(d, layer, nicp, s_var[2], Tl, cl, wsum, sys + 27);
+                }
                 if (wave == 0) {
                     SolveCfg sc;
-                    sc.solver = cfg.solver; sc.mode = 1; sc.lambda = 0.0f; sc.scale_out = cfg.scale_out;
+                    sc.solver = cfg.solver; sc.mode = MODE; sc.lambda = cfg.lambda; sc.scale_out = cfg.scale_out;
                     sc.robust = cfg.robust; sc.mad_bins = cfg.mad_bins; sc.scale_in = 1e-3f;
-                    const bool ok = solve_update_wave(sys, T, sc, N, 1, ws);
-                    if (t == 0) {
-                        if (!ok) s_ok = 0;
-                        rms_last = sys[28] > 0.0 ? (float)sqrt(sys[27] / sys[28]) : 0.0f;
+                    // no update from an empty geometric selection (as the single-pair path)
+                    if (MODE == 1 || nicp > 0) {
+                        const bool ok = solve_update_wave(sys, T, sc, N, nicp, ws);
+                        if (t == 0 && !ok) s_ok = 0;
                     }
+                    if (t == 0 && MODE != 0)
+                        rms_last = sys[RMS0 + 1] > 0.0 ? (float)sqrt(sys[RMS0] / sys[RMS0 + 1]) : 0.0f;
                 }
                 __syncthreads();
             }
@@ -656,7 +857,7 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
 }
 
 typedef void (*TrackFn)(const PairDesc*, const int*, int, int*, TrackCfg);
-struct Variant { int bs, ppt, rz, wpe; TrackFn fn; int pc; };
+struct Variant { int bs, ppt, rz, wpe; TrackFn fn; int pc = 0; int mode = RT_PHOTOMETRIC; };
 static const Variant kVariants[] = {
     {256, 8, 0, 1, kb_track<256, 8, 0>},     {256, 16, 0, 1, kb_track<256, 16, 0>},
     {256, 32, 0, 1, kb_track<256, 32, 0>},   {512, 4, 0, 1, kb_track<512, 4, 0>},
@@ -675,6 +876,18 @@ static const Variant kVariants[] = {
     {512, 16, 0, 1, kb_track<512, 16, 0, 1, 1>, 8192},
     {1024, 2, 1, 1, kb_track<1024, 2, 1, 1, 1>, 5632},   {1024, 4, 1, 1, kb_track<1024, 4, 1, 1, 1>, 5632},
     {512, 4, 1, 1, kb_track<512, 4, 1, 1, 1>, 5632},     {1024, 2, 1, 1, kb_track<1024, 2, 1, 1, 1>, 5120},
+    // bi-objective: the photometric variants' shapes with the geometric terms
+    // (the selection's LDS leaves room for a smaller point cache)
+    {1024, 2, 1, 1, kb_track<1024, 2, 1, 1, 1, RT_BIOBJECTIVE>, 4608, RT_BIOBJECTIVE},
+    {1024, 4, 1, 1, kb_track<1024, 4, 1, 1, 1, RT_BIOBJECTIVE>, 4608, RT_BIOBJECTIVE},
+    {1024, 2, 1, 1, kb_track<1024, 2, 1, 1, 0, RT_BIOBJECTIVE>, 0, RT_BIOBJECTIVE},
+    {1024, 8, 0, 1, kb_track<1024, 8, 0, 1, 0, RT_BIOBJECTIVE>, 0, RT_BIOBJECTIVE},
+    {512, 4, 1, 1, kb_track<512, 4, 1, 1, 0, RT_BIOBJECTIVE>, 0, RT_BIOBJECTIVE},
+    {256, 4, 1, 1, kb_track<256, 4, 1, 1, 0, RT_BIOBJECTIVE>, 0, RT_BIOBJECTIVE},
+    // geometric only: no photometric points (ppt and rz unused)
+    {1024, 1, 0, 1, kb_track<1024, 1, 0, 1, 0, RT_GEOMETRIC>, 0, RT_GEOMETRIC},
+    {512, 1, 0, 1, kb_track<512, 1, 0, 1, 0, RT_GEOMETRIC>, 0, RT_GEOMETRIC},
+    {256, 1, 0, 1, kb_track<256, 1, 0, 1, 0, RT_GEOMETRIC>, 0, RT_GEOMETRIC},
 };
 
 }  // namespace rt
@@ -722,18 +935,23 @@ struct BatchMem {
     size_t pair_bytes;
     int max_w, max_h;
     size_t nsel;
+    bool icp;         // geometric buffers in the arena (geometric and bi-objective modes)
     float* calib;     // 96 per pair
     float* trel_in;   // 16 per pair
     float* trel_out;  // 16 per pair
     float* rms;
     int* status;
 };
-static size_t pair_arena_bytes(int max_w, int max_h, size_t N) {
+static size_t pair_arena_bytes(int max_w, int max_h, size_t N, bool icp = false) {
     const size_t npx = (size_t)max_w * max_h;
     size_t per = 0;  // inputs are borrowed device pointers; the arena holds derived data only
     per += 2 * align256(npx * 4 / 3 + 64);      // two gray pyramids, levels 1..3
     per += align256(npx);                       // gradient-magnitude bins
     per += align256(256 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(jac_bytes((int)N));
+    if (icp) {
+        for (int l = 0; l < 4; l++) per += 3 * align256((npx >> (2 * l)) * 16);  // rpos, rnrm, cpos
+        per += 2 * align256(npx * 4) + align256(64);                                // isel, ir2, statistics
+    }
     return per;
 }
 
@@ -752,7 +970,9 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) b->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return set_error(RT_ERR_HIP, "stream"); }
-    b->pair_bytes = pair_arena_bytes(max_w, max_h, (size_t)b->prm.n_select);
+    if (b->prm.mode < RT_GEOMETRIC || b->prm.mode > RT_BIOBJECTIVE) { delete b; return set_error(RT_ERR_ARG, "bad mode"); }
+    b->prm.lambda = std::min(1.0f, std::max(0.0f, b->prm.lambda));  // setLambda's clamp (phototracker.cpp:163-168)
+    b->pair_bytes = pair_arena_bytes(max_w, max_h, (size_t)b->prm.n_select, b->prm.mode != RT_PHOTOMETRIC);
     size_t total = b->pair_bytes * (size_t)max_pairs;
     void* q = nullptr;
     if (hipMalloc(&q, total) != hipSuccess) { hipStreamDestroy(b->stream); delete b; return set_error(RT_ERR_NOMEM, "batch arena"); }
@@ -799,10 +1019,12 @@ struct BatchView {
     int* queue;
     int n, maxW, maxH, maxN;
     int solver, robust, mad_bins;
+    int mode;
+    float lambda;
 };
 static BatchView view_of(const rt_batch* b) {
     return BatchView{b->desc_dev, b->order_dev, b->queue, b->n, b->maxW, b->maxH, b->maxN,
-                     b->prm.solver, b->prm.robust, b->prm.mad_bins};
+                     b->prm.solver, b->prm.robust, b->prm.mad_bins, b->prm.mode, b->prm.lambda};
 }
 
 static void launch_prep(const BatchView& v, hipStream_t s) {
@@ -814,10 +1036,16 @@ static void launch_prep(const BatchView& v, hipStream_t s) {
     kb_bins<<<gb, 256, 0, s>>>(v.desc);
     kb_select<<<n, SEL_BS, 0, s>>>(v.desc);
     kb_ref_points<<<dim3((v.maxN + 255) / 256, n), 256, 0, s>>>(v.desc);
+    if (v.mode != RT_PHOTOMETRIC) {
+        kb_clouds<<<dim3((v.maxW + 15) / 16, (v.maxH + 15) / 16, n), 256, 0, s>>>(v.desc);
+        for (int l = 1; l < 4; l++)
+            kb_clouds_down<<<dim3(((v.maxW >> l) * (v.maxH >> l) + 255) / 256, n), 256, 0, s>>>(v.desc, l);
+    }
 }
 
 static TrackCfg track_cfg(const BatchView& bv, const Variant& v) {
     TrackCfg c;
+    c.lambda = bv.lambda;
     c.pc_cap = v.pc;
     c.rz_len = bv.maxN;
     c.solver = bv.solver;
@@ -829,7 +1057,7 @@ static TrackCfg track_cfg(const BatchView& bv, const Variant& v) {
 
 static size_t track_dyn_lds(const Variant& v, int maxN) {
     size_t n = 0;
-    if (v.rz) n += sizeof(float2) * (size_t)maxN;
+    if (v.rz && v.mode != RT_GEOMETRIC) n += sizeof(float2) * (size_t)maxN;
     if (v.pc) n += sizeof(float4) * (size_t)std::min(maxN, v.pc);
     return n;
 }
@@ -888,7 +1116,10 @@ static int autotune(rt_batch* b) {
     int bv = -1, bg = 0;
     for (int v = 0; v < NUM_VARIANTS; v++) {
         const Variant& var = kVariants[v];
-        if (var.rz) {
+        if (var.mode != b->prm.mode) continue;
+        if (var.mode == RT_GEOMETRIC) {
+            if (!lds_variant_ready(var, b->maxN)) continue;  // no photometric points to cover
+        } else if (var.rz) {
             if (!lds_variant_ready(var, b->maxN)) continue;
         } else {
             if (var.bs * var.ppt < b->maxN) continue;
@@ -928,11 +1159,11 @@ static int autotune(rt_batch* b) {
 
 // The kernel variant table, for tests that force each variant through
 // RT_BATCH_VARIANT: info = {workgroup size, points per thread (unroll depth
-// for rz), residuals in LDS, point-cache capacity, waves per SIMD}.
-extern "C" int rt_batch_variant_info(int v, int info[5]) {
+// for rz), residuals in LDS, point-cache capacity, waves per SIMD, mode}.
+extern "C" int rt_batch_variant_info(int v, int info[6]) {
     if (!info || v < 0 || v >= NUM_VARIANTS) return set_error(RT_ERR_ARG, "rt_batch_variant_info: no such variant");
     const Variant& k = kVariants[v];
-    info[0] = k.bs; info[1] = k.ppt; info[2] = k.rz; info[3] = k.pc; info[4] = k.wpe;
+    info[0] = k.bs; info[1] = k.ppt; info[2] = k.rz; info[3] = k.pc; info[4] = k.wpe; info[5] = k.mode;
     return RT_OK;
 }
 
@@ -1028,6 +1259,19 @@ static int build_batch(const rt_params& prm, const BatchMem& m, int n, const rt_
         d.sel = (int*)(base + off); off += align256(NN * 4);
         d.pts = (float4*)(base + off); off += align256(NN * 16);
         for (int l = 0; l < 4; l++) { d.jac[l] = (float4*)(base + off); off += align256(jac_bytes((int)NN)); }
+        if (m.icp) {
+            const size_t mpx = (size_t)m.max_w * m.max_h;
+            for (int l = 0; l < 4; l++) {
+                const size_t sz = align256((mpx >> (2 * l)) * 16);
+                d.rpos[l] = (float4*)(base + off); off += sz;
+                d.rnrm[l] = (float4*)(base + off); off += sz;
+                d.cpos[l] = (float4*)(base + off); off += sz;
+            }
+            d.isel = (int*)(base + off); off += align256(mpx * 4);
+            d.ir2 = (float*)(base + off); off += align256(mpx * 4);
+            d.icnt = (int*)(base + off);
+            d.ivar = (float*)(base + off + 16); off += align256(64);
+        }
         d.trel_in = m.trel_in + 16 * (size_t)i;
         d.trel_out = m.trel_out + 16 * (size_t)i;
         d.rms = m.rms + i;
@@ -1049,7 +1293,7 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
     std::vector<float> calib(96 * (size_t)n);
     std::vector<float> trel(16 * (size_t)n);
     std::vector<int> order(n);
-    const BatchMem m{b->arena, b->pair_bytes, b->max_w, b->max_h, (size_t)b->prm.n_select,
+    const BatchMem m{b->arena, b->pair_bytes, b->max_w, b->max_h, (size_t)b->prm.n_select, b->prm.mode != RT_PHOTOMETRIC,
                      b->calib_dev, b->trel_in, b->trel_out, b->rms, b->status};
     int maxW = 0, maxH = 0, maxN = 0;
     int rc = build_batch(b->prm, m, n, pairs, desc.data(), calib.data(), trel.data(), order.data(), maxW, maxH, maxN);
@@ -1089,7 +1333,9 @@ extern "C" int rt_batch_propose(rt_batch* b, rt_cnn* net) {
 
 // Inspection of one pair of the last prepared batch: RT_DBG_GRAD_HIST,
 // RT_DBG_SELECTED, RT_DBG_POINTS, RT_DBG_JACOBIAN (level), RT_DBG_TREL (initial
-// pose of the pair).
+// pose of the pair); in the geometric modes RT_DBG_REF_CLOUD, RT_DBG_CUR_CLOUD
+// (level) and, after tracking, RT_DBG_ICP_COUNT (the four levels' counts) and
+// RT_DBG_ICP_VAR (level: the three Student-t sums).
 extern "C" int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, void* host, size_t bytes) {
     if (!b || !host || pair < 0 || pair >= b->n) return set_error(RT_ERR_ARG, "rt_batch_debug_read: bad argument");
     if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
@@ -1104,6 +1350,26 @@ extern "C" int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, v
             if (level < 0 || level >= d.L) return set_error(RT_ERR_ARG, "bad level");
             src = d.jac[level]; n = jac_bytes(d.N); break;
         case RT_DBG_TREL: src = d.trel_in; n = 64; break;
+        case RT_DBG_REF_CLOUD:
+        case RT_DBG_CUR_CLOUD:
+        case RT_DBG_ICP_COUNT:
+        case RT_DBG_ICP_VAR: {
+            if (!d.isel) return set_error(RT_ERR_STATE, "rt_batch_debug_read: geometric buffers exist in the geometric modes only");
+            if (level < 0 || level >= d.L) return set_error(RT_ERR_ARG, "bad level");
+            const size_t nl = (size_t)(d.W >> level) * (d.H >> level);
+            if (what == RT_DBG_REF_CLOUD) {
+                if (bytes < nl * 32) return set_error(RT_ERR_ARG, "buffer too small");
+                if (hipMemcpyAsync(host, d.rpos[level], nl * 16, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+                    hipMemcpyAsync((char*)host + nl * 16, d.rnrm[level], nl * 16, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+                    hipStreamSynchronize(b->stream) != hipSuccess)
+                    return set_error(RT_ERR_HIP, "rt_batch_debug_read: copy");
+                return RT_OK;
+            }
+            if (what == RT_DBG_CUR_CLOUD) { src = d.cpos[level]; n = nl * 16; }
+            else if (what == RT_DBG_ICP_COUNT) { src = d.icnt; n = 16; }
+            else { src = d.ivar + 3 * level; n = 12; }
+            break;
+        }
         default: return set_error(RT_ERR_UNSUPPORTED, "rt_batch_debug_read: not available for batches");
     }
     if (bytes < n) return set_error(RT_ERR_ARG, "buffer too small");
@@ -1181,7 +1447,7 @@ int noalloc_variant(int maxN) {
     int fallback = -1;
     for (int v = 0; v < NUM_VARIANTS; v++) {
         const rt::Variant& k = rt::kVariants[v];
-        if (k.bs != 1024) continue;
+        if (k.bs != 1024 || k.mode != RT_PHOTOMETRIC) continue;
         if (k.rz && k.pc && k.ppt == 4 && lds_variant_ready(k, maxN)) return v;
         if (!k.rz && !k.pc && k.bs * k.ppt >= maxN && fallback < 0) fallback = v;
     }
@@ -1203,6 +1469,7 @@ extern "C" int rt_batch_track_noalloc(int device, int n, const rt_pair* pairs, c
     if (prm.n_select < 256 || prm.n_select % 256 || prm.n_select > RT_NOALLOC_MAX_POINTS)
         return set_error(RT_ERR_ARG, "rt_batch_track_noalloc: n_select must be a multiple of 256 up to RT_NOALLOC_MAX_POINTS");
     if (prm.mad_bins != 64 && prm.mad_bins != 256) return set_error(RT_ERR_ARG, "mad_bins must be 64 or 256");
+    if (prm.mode != RT_PHOTOMETRIC) return set_error(RT_ERR_UNSUPPORTED, "rt_batch_track_noalloc: photometric mode only");
     for (int i = 0; i < n; i++)
         if (pairs[i].width > RT_NOALLOC_MAX_WIDTH || pairs[i].height > RT_NOALLOC_MAX_HEIGHT)
             return set_error(RT_ERR_ARG, "rt_batch_track_noalloc: pair larger than RT_NOALLOC_MAX_WIDTH x RT_NOALLOC_MAX_HEIGHT");
@@ -1212,7 +1479,7 @@ extern "C" int rt_batch_track_noalloc(int device, int n, const rt_pair* pairs, c
     int rc = noalloc_scratch(device, &base);
     if (rc) return rc;
     const BatchMem m{base + NA_TABLES, NA_PAIR_BYTES, RT_NOALLOC_MAX_WIDTH, RT_NOALLOC_MAX_HEIGHT,
-                     (size_t)RT_NOALLOC_MAX_POINTS, (float*)(base + NA_CALIB), (float*)(base + NA_TREL_IN),
+                     (size_t)RT_NOALLOC_MAX_POINTS, false, (float*)(base + NA_CALIB), (float*)(base + NA_TREL_IN),
                      (float*)(base + NA_TREL_OUT), (float*)(base + NA_RMS), (int*)(base + NA_STATUS)};
     // host tables in this frame; the stream is synchronised before return, so
     // the queued copies have read them by then
@@ -1234,7 +1501,7 @@ extern "C" int rt_batch_track_noalloc(int device, int n, const rt_pair* pairs, c
         return set_error(RT_ERR_HIP, "rt_batch_track_noalloc: table upload");
     }
     const BatchView v{(const PairDesc*)(base + NA_DESC), (const int*)(base + NA_ORDER), (int*)(base + NA_QUEUE),
-                      n, maxW, maxH, maxN, prm.solver, prm.robust, prm.mad_bins};
+                      n, maxW, maxH, maxN, prm.solver, prm.robust, prm.mad_bins, prm.mode, prm.lambda};
     (void)hipGetLastError();
     launch_prep(v, s);
     launch_track(v, variant, std::min(n, cus), s);
@@ -1263,9 +1530,11 @@ extern "C" int rt_batch_track_noalloc(int device, int n, const rt_pair* pairs, c
 // The descriptors rt_batch_prepare would build for n pairs in a batch of
 // max_width x max_height, as offsets from the batch's arena (host code only:
 // no device is touched), for bounds checks of the launches on the CPU. Per pair
-// RT_LAYOUT_FIELDS int64 values: W, H, L, N, simple, then the byte offsets of
-// gray[0][1..3], gray[1][1..3], bins, ghist, sel, pts, jac[0..3] from the
-// arena, the pair's arena slot start, and the slot size; then the work order.
+// 40 int64 values: W, H, L, N, simple, then the byte offsets of gray[0][1..3],
+// gray[1][1..3], bins, ghist, sel, pts, jac[0..3] from the arena, the pair's
+// arena slot start and the slot size, then (geometric modes, else -1) the
+// offsets of rpos, rnrm, cpos of levels 0..3, isel, ir2, icnt, ivar; then the
+// work order.
 extern "C" int rt_batch_layout(const rt_params* params, int max_w, int max_h, int n, const rt_pair* pairs,
                                int64_t* fields, int* order, int maxes[3]) {
     if (!pairs || !fields || !order || !maxes || n < 1) return set_error(RT_ERR_ARG, "rt_batch_layout: bad argument");
@@ -1273,8 +1542,9 @@ extern "C" int rt_batch_layout(const rt_params* params, int max_w, int max_h, in
     if (params) prm = *params; else rt_default_params(&prm);
     // a nominal arena address: only offsets from it are reported
     char* const arena = reinterpret_cast<char*>((uintptr_t)1 << 40);
-    const size_t pb = pair_arena_bytes(max_w, max_h, (size_t)prm.n_select);
-    const BatchMem m{arena, pb, max_w, max_h, (size_t)prm.n_select, nullptr, nullptr, nullptr, nullptr, nullptr};
+    const bool icp = prm.mode != RT_PHOTOMETRIC;
+    const size_t pb = pair_arena_bytes(max_w, max_h, (size_t)prm.n_select, icp);
+    const BatchMem m{arena, pb, max_w, max_h, (size_t)prm.n_select, icp, nullptr, nullptr, nullptr, nullptr, nullptr};
     std::vector<PairDesc> desc(n);
     std::vector<float> calib(96 * (size_t)n), trel(16 * (size_t)n);
     int rc = build_batch(prm, m, n, pairs, desc.data(), calib.data(), trel.data(), order, maxes[0], maxes[1], maxes[2]);
@@ -1282,13 +1552,20 @@ extern "C" int rt_batch_layout(const rt_params* params, int max_w, int max_h, in
     auto off = [&](const void* p) { return p ? (int64_t)((const char*)p - arena) : (int64_t)-1; };
     for (int i = 0; i < n; i++) {
         const PairDesc& d = desc[i];
-        int64_t* f = fields + (size_t)i * 24;
+        int64_t* f = fields + (size_t)i * 40;
         f[0] = d.W; f[1] = d.H; f[2] = d.L; f[3] = d.N; f[4] = d.simple;
         for (int l = 1; l < 4; l++) { f[4 + l] = off(d.gray[0][l]); f[7 + l] = off(d.gray[1][l]); }
         f[11] = off(d.bins); f[12] = off(d.ghist); f[13] = off(d.sel); f[14] = off(d.pts);
         for (int l = 0; l < 4; l++) f[15 + l] = off(d.jac[l]);
         f[19] = (int64_t)(pb * (size_t)i); f[20] = (int64_t)pb;
-        f[21] = f[22] = f[23] = 0;
+        for (int l = 0; l < 4; l++) {
+            f[21 + 3 * l] = icp ? off(d.rpos[l]) : -1;
+            f[22 + 3 * l] = icp ? off(d.rnrm[l]) : -1;
+            f[23 + 3 * l] = icp ? off(d.cpos[l]) : -1;
+        }
+        f[33] = icp ? off(d.isel) : -1; f[34] = icp ? off(d.ir2) : -1;
+        f[35] = icp ? off(d.icnt) : -1; f[36] = icp ? off(d.ivar) : -1;
+        f[37] = f[38] = f[39] = 0;
     }
     return RT_OK;
 }

@@ -730,4 +730,163 @@ __device__ __forceinline__ float robust_weight(float r, float zw, float s, int r
     return w;
 }
 
+
+// ------------------------------------------------------------------ A18-A23 per pixel / point
+// convertZMapToXYZ6Kernel (rgb2gray.cu:595-624) + computeNormalsCudaKernel
+// (d2z.cu:503-537) for pixel (x, y): position (x, y, z, mask) and normal from
+// the depth source zs (z < 0 valid).
+template <class ZS>
+__device__ __forceinline__ void ref_cloud_pixel(const ZS& zs, int W, int H, int x, int y, const float* __restrict__ c,
+                                                float4* __restrict__ pos, float4* __restrict__ nrm) {
+    size_t o = (size_t)x + (size_t)y * W;
+    f3 p;
+    bool valid = backproject(zs[o], x, y, c, p);
+    float mask = valid ? 1.0f : 0.0f;
+    float4 n = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (x < 1 || y < 1 || x > W - 2 || y > H - 2) {
+        mask = 0.0f;
+    } else {
+        f3 a, b2, cc, dd;
+        backproject(zs[o - 1], x - 1, y, c, a);
+        backproject(zs[o + 1], x + 1, y, c, b2);
+        backproject(zs[o - W], x, y - 1, c, cc);
+        backproject(zs[o + W], x, y + 1, c, dd);
+        if (a.z < 0.0f && b2.z < 0.0f && cc.z < 0.0f && dd.z < 0.0f) {
+            f3 u{b2.x - a.x, b2.y - a.y, b2.z - a.z};
+            f3 v{dd.x - cc.x, dd.y - cc.y, dd.z - cc.z};
+            float nx = u.y * v.z - u.z * v.y;
+            float ny = -(u.x * v.z - u.z * v.x);
+            float nz = u.x * v.y - u.y * v.x;
+            float len = nx * nx + ny * ny + nz * nz;
+            len = 1.0f / sqrtf(len);
+            n = make_float4(nx * len, ny * len, nz * len, 0.0f);
+        } else {
+            mask = 0.0f;
+        }
+    }
+    pos[o] = make_float4(p.x, p.y, p.z, mask);
+    nrm[o] = n;
+}
+// genPointsCloud6Kernel (d2z.cu:436-469) for pixel i: every z < 0 is valid.
+template <class ZS>
+__device__ __forceinline__ void cur_cloud_pixel(const ZS& zs, int W, int i, const float* __restrict__ c,
+                                                float4* __restrict__ pos) {
+    int x = i % W, y = i / W;
+    float zz = zs[(size_t)i];
+    if (zz < 0.0f) {
+        float fx = c[KL], fy = c[KL + 4], cx = c[KL + 2], cy = c[KL + 5];
+        pos[i] = make_float4(((float)x - cx) * zz / fx, ((float)y - cy) * zz / fy, zz, 1.0f);
+    } else {
+        pos[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+// downSampleCloud6Kernel (d2z.cu:557-588) for destination pixel i of a
+// Wd x Hd level: masked 2x2 mean, normal renormalised (sn / dn may be null).
+__device__ __forceinline__ void cloud_down_pixel(const float4* __restrict__ sp, const float4* __restrict__ sn, int Wd,
+                                                 int i, float4* __restrict__ dp, float4* __restrict__ dn) {
+    int x = i % Wd, y = i / Wd;
+    int Ws = Wd * 2;
+    size_t h = (size_t)x * 2 + (size_t)y * 2 * Ws;
+    size_t hs[4] = {h, h + 1, h + Ws, h + Ws + 1};
+    float d[6] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    float n = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        float4 p = sp[hs[t]];
+        if (p.w != 0.0f) {
+            n++;
+            d[0] += p.x; d[1] += p.y; d[2] += p.z;
+            if (sn) { float4 q = sn[hs[t]]; d[3] += q.x; d[4] += q.y; d[5] += q.z; }
+        }
+    }
+    if (n > 0.0f) {
+        for (int k = 0; k < 6; k++) d[k] /= n;
+        dp[i] = make_float4(d[0], d[1], d[2], 1.0f);
+        if (dn) {
+            float len = d[3] * d[3] + d[4] * d[4] + d[5] * d[5];
+            len = 1.0f / sqrtf(len);
+            dn[i] = make_float4(d[3] * len, d[4] * len, d[5] * len, 0.0f);
+        }
+    } else {
+        dp[i] = make_float4(d[0], d[1], d[2], 0.0f);
+        if (dn) dn[i] = make_float4(d[3], d[4], d[5], 0.0f);
+    }
+}
+// icpResidualKernel2 (rgb2gray.cu:223-260) for keyframe pixel o of a w x h
+// level (a = 2^-l, b = (a - 1) / 2): valid when the keyframe point exists and
+// the four current taps it projects to are valid; r = (bilerp(c) - T p) . (R n).
+__device__ __forceinline__ bool icp_mask_pixel(const float4* __restrict__ rp, const float4* __restrict__ rn,
+                                               const float4* __restrict__ cp, int w, int h, int o, const float* T,
+                                               const float* c, float a, float b, float& r) {
+    float4 p = rp[o];
+    if (p.w == 0.0f) return false;
+    float4 nn = rn[o];
+    f3 p3{p.x, p.y, p.z}, n3{nn.x, nn.y, nn.z};
+    f3 q3 = mul4(T, p3);
+    f3 cn = rot4(T, n3);
+    const float* K = c + KL;
+    float qx = q3.x / q3.z, qy = q3.y / q3.z;
+    float px = a * (K[0] * qx + K[2]) + b;
+    float py = a * (K[4] * qy + K[5]) + b;
+    if (!(px >= 0.0f && px < (float)(w - 1) && py >= 0.0f && py < (float)(h - 1))) return false;
+    int xdi = (int)px, ydi = (int)py;
+    int o2 = xdi + ydi * w;
+    float4 c0 = cp[o2], c1 = cp[o2 + 1], c2 = cp[o2 + w], c3 = cp[o2 + w + 1];
+    if (!(c0.w != 0.0f && c1.w != 0.0f && c2.w != 0.0f && c3.w != 0.0f)) return false;
+    float fx = px - (float)xdi, fy = py - (float)ydi;
+    f3 cc{blend(c0.x, c1.x, c2.x, c3.x, fx, fy), blend(c0.y, c1.y, c2.y, c3.y, fx, fy),
+          blend(c0.z, c1.z, c2.z, c3.z, fx, fy)};
+    f3 dd{cc.x - q3.x, cc.y - q3.y, cc.z - q3.z};
+    r = dd.x * cn.x + dd.y * cn.y + dd.z * cn.z;
+    return true;
+}
+// icpResidualIndexKernel2 (rgb2gray.cu:323-390) for keyframe pixel o with its
+// squared mask residual x2, the final Student-t weight (clamped to 1) from the
+// variance sum v3 over n points: scaled residual and the 6 Jacobian entries
+// (zeros when the point leaves the view, its first tap is invalid, the weight
+// vanishes or |dz| >= 300 mm).
+__device__ __forceinline__ void icp_point_terms(const float4* __restrict__ rp, const float4* __restrict__ rn,
+                                                const float4* __restrict__ cp, int w, int h, int o, float x2, float v3,
+                                                int n, const float* T, const float* c, float a, float b, float s,
+                                                float& res, float* J) {
+    float4 p = rp[o], nn = rn[o];
+    f3 p3{p.x, p.y, p.z}, rnv{nn.x, nn.y, nn.z};
+    f3 q3 = mul4(T, p3);
+    f3 cn = rot4(T, rnv);
+    const float* K = c + KL;
+    float qx = q3.x / q3.z, qy = q3.y / q3.z;
+    float px = a * (K[0] * qx + K[2]) + b;
+    float py = a * (K[4] * qy + K[5]) + b;
+    res = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; k++) J[k] = 0.0f;
+    if (px >= 0.0f && px < (float)(w - 1) && py >= 0.0f && py < (float)(h - 1)) {
+        float wt = fminf(6 / (5 + x2 / (v3 / (float)n)), 1.0f);
+        int xdi = (int)px, ydi = (int)py;
+        float fx = px - (float)xdi, fy = py - (float)ydi;
+        int o2 = xdi + ydi * w;
+        float4 c0 = cp[o2], c1 = cp[o2 + 1], c2 = cp[o2 + w], c3 = cp[o2 + w + 1];
+        f3 cc{blend(c0.x, c1.x, c2.x, c3.x, fx, fy), blend(c0.y, c1.y, c2.y, c3.y, fx, fy),
+              blend(c0.z, c1.z, c2.z, c3.z, fx, fy)};
+        if (c0.w != 0.0f && wt > 0.0f && fabsf(cc.z - q3.z) < 300.0f) {
+            f3 d{cc.x - q3.x, cc.y - q3.y, cc.z - q3.z};
+            res = s * wt * (d.x * cn.x + d.y * cn.y + d.z * cn.z);
+            f3 gp[3] = {{0.0f, -p3.z, p3.y}, {p3.z, 0.0f, -p3.x}, {-p3.y, p3.x, 0.0f}};
+            f3 gn[3] = {{0.0f, -rnv.z, rnv.y}, {rnv.z, 0.0f, -rnv.x}, {-rnv.y, rnv.x, 0.0f}};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                f3 dB = rot4(T, gp[k]);
+                f3 dN = rot4(T, gn[k]);
+                J[k] = -s * wt * (-cn.x * dB.x - cn.y * dB.y - cn.z * dB.z + d.x * dN.x + d.y * dN.y + d.z * dN.z);
+            }
+            f3 e[3] = {{1.0f, 0.0f, 0.0f}, {0.0f, 1.0f, 0.0f}, {0.0f, 0.0f, 1.0f}};
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                f3 dB = rot4(T, e[k]);
+                J[3 + k] = -wt * (-cn.x * dB.x - cn.y * dB.y - cn.z * dB.z);
+            }
+        }
+    }
+}
+
 }  // namespace rt

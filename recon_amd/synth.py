@@ -142,3 +142,42 @@ def pose_error(T_est, T_gt):
     s = 0.5 * np.linalg.norm([R[2, 1] - R[1, 2], R[0, 2] - R[2, 0], R[1, 0] - R[0, 1]])
     c = (np.trace(R) - 1.0) / 2.0
     return float(np.degrees(np.arctan2(s, c))), float(np.linalg.norm(D[:3, 3]))
+
+
+MIXED_SIZES = [(160, 120), (320, 240), (480, 360), (640, 480)]
+
+
+def render_mixed_batch(n=256, seed=3, theta=3.0, tmax=50.0):
+    """The mixed-size batch workload: n frame pairs with sizes drawn uniformly
+    from MIXED_SIZES (numpy default_rng(seed)), one scene per pair, motions <=
+    theta deg / tmax mm, rendered on the GPU into one gray and one depth
+    buffer; the device is synchronised before return (the frames are rendered
+    on the null stream, which a tracker's non-blocking stream does not wait
+    for). Returns (pairs, gray buffer, depth buffer): per pair the rt_pair
+    fields, its true Trel ("gt") and its two frames' pixel offsets ("off")."""
+    from .hip import DeviceBuffer, hip
+    S = synth_lib()
+    sizes = [MIXED_SIZES[k] for k in np.random.default_rng(seed).integers(0, len(MIXED_SIZES), n)]
+    npx = [W * H for W, H in sizes]
+    off = np.concatenate([[0], np.cumsum([2 * p for p in npx])])
+    g, d = DeviceBuffer(int(off[-1])), DeviceBuffer(int(off[-1]) * 2)
+    pairs = []
+    for i, (W, H) in enumerate(sizes):
+        sc = S.rs_scene_create(ctypes.c_uint64(seed * 10000 + i))
+        M = sample_motion(seed * 10000 + 7000 + i, theta, tmax)
+        fx, fy, cx, cy = pinhole(W, H)
+        for f, T in ((0, np.eye(4)), (1, M)):
+            Tf = np.ascontiguousarray(T, np.float64).ravel()
+            o = int(off[i]) + f * npx[i]
+            rc = S.rs_render_device(sc, Tf.ctypes.data_as(_dp), W, H, fx, fy, cx, cy,
+                                    ctypes.c_uint64(seed * 100000 + 2 * i + f), 0.01, g.ptr + o, d.ptr + 2 * o, None)
+            if rc != 0:
+                raise RuntimeError(f"render failed: {rc}")
+        S.rs_scene_destroy(sc)
+        o0, o1 = int(off[i]), int(off[i]) + npx[i]
+        pairs.append(dict(ref_gray=g.ptr + o0, ref_depth=d.ptr + 2 * o0, cur_gray=g.ptr + o1, cur_depth=d.ptr + 2 * o1,
+                          width=W, height=H, calib=calib_block_np(W, H), gt=np.linalg.inv(M), off=(o0, o1)))
+    rc = hip().hipDeviceSynchronize()
+    if rc != 0:
+        raise RuntimeError(f"hipDeviceSynchronize failed: {rc}")
+    return pairs, g, d

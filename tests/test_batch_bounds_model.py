@@ -17,7 +17,7 @@ from recon_amd import RtPair, lib
 from recon_amd.tracker import default_params
 
 SIZES = [(160, 120), (320, 240), (480, 360), (640, 480)]
-FIELDS = 24
+FIELDS = 40
 BINS_TILE_H = 16
 NW_BINS = ((64 + 8) // 4) * (BINS_TILE_H + 4)   # words of the kb_bins tile (TWW * TH)
 
@@ -26,7 +26,7 @@ def _align(x):
     return (x + 255) & ~255
 
 
-def _layout(sizes, N, max_w=640, max_h=480):
+def _layout(sizes, N, max_w=640, max_h=480, mode=1):
     n = len(sizes)
     arr = (RtPair * n)()
     cal = np.zeros(96, np.float32)
@@ -43,7 +43,7 @@ def _layout(sizes, N, max_w=640, max_h=480):
     f = np.zeros(n * FIELDS, np.int64)
     order = np.zeros(n, np.int32)
     maxes = np.zeros(3, np.int32)
-    prm = default_params(n_select=N)
+    prm = default_params(n_select=N, mode=mode)
     rc = lib().rt_batch_layout(ctypes.byref(prm), max_w, max_h, n, arr, f.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                maxes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
@@ -70,15 +70,30 @@ def _buffers(p, max_w, max_h):
     b["pts"] = (int(p[14]), N * 16)
     for l in range(4):
         b[f"jac{l}"] = (int(p[15 + l]), N * 28)
+    if p[21] >= 0:   # geometric buffers, sized by the pair's own levels
+        for l in range(4):
+            nl = (W >> l) * (H >> l)
+            for k, name in enumerate(("rpos", "rnrm", "cpos")):
+                b[f"{name}{l}"] = (int(p[21 + 3 * l + k]), nl * 16)
+        b["isel"] = (int(p[33]), W * H * 4)
+        b["ir2"] = (int(p[34]), W * H * 4)
+        b["icnt"] = (int(p[35]), 16)
+        b["ivar"] = (int(p[36]), 48)
+        assert p[36] == p[35] + 16
+    else:
+        assert np.all(p[21:37] == -1)
     return b
 
 
-def _check_layout(f, order, maxes, max_w, max_h, N):
+def _check_layout(f, order, maxes, max_w, max_h, N, icp=False):
     n = len(f)
     slot = int(f[0, 20])
     npx = max_w * max_h
-    assert slot == 2 * _align(npx * 4 // 3 + 64) + _align(npx) + _align(1024) + _align(N * 4) + _align(N * 16) + \
+    want = 2 * _align(npx * 4 // 3 + 64) + _align(npx) + _align(1024) + _align(N * 4) + _align(N * 16) + \
         4 * _align(N * 28)
+    if icp:
+        want += sum(3 * _align((npx >> (2 * l)) * 16) for l in range(4)) + 2 * _align(npx * 4) + _align(64)
+    assert slot == want
     assert sorted(order.tolist()) == list(range(n))         # the work queue hands out every pair once
     assert maxes[0] == f[:, 0].max() and maxes[1] == f[:, 1].max() and maxes[2] == f[:, 3].max()
     for i in range(n):
@@ -172,6 +187,31 @@ def _kb_track(W, H, L, N, maxN, pc):
     return out
 
 
+def _geometric(W, H, L):
+    """kb_clouds, kb_clouds_down and the geometric terms of kb_track: largest
+    element index per buffer (float4 clouds, isel / ir2)."""
+    out = {}
+    # kb_clouds: 16x16 pixel tiles; the normal's four neighbours only for
+    # 1 <= x <= W - 2, 1 <= y <= H - 2
+    out["depth"] = max((W - 1) + (H - 1) * W, (W - 2 + 1) + (H - 2) * W, (W - 2) + (H - 2 + 1) * W)
+    out["cloud0"] = W * H - 1
+    for l in range(1, L):
+        Wd, Hd = W >> l, H >> l
+        assert (W >> (l - 1)) == 2 * Wd and (H >> (l - 1)) == 2 * Hd
+        i = Wd * Hd - 1
+        x, y = i % Wd, i // Wd
+        src = x * 2 + 1 + (y * 2 + 1) * 2 * Wd              # the last 2x2 source tap
+        out[f"down{l}"] = (src, (W >> (l - 1)) * (H >> (l - 1)) - 1)
+    for l in range(L):
+        w, h = W >> l, H >> l
+        # mask / system taps: 0 <= xdi < w - 1, 0 <= ydi < h - 1, taps o2 .. o2 + w + 1
+        out[f"taps{l}"] = ((w - 2) + (h - 2) * w + w + 1, w * h - 1)
+        out[f"isel{l}"] = (w * h - 1, W * H - 1)               # at most w*h selected points
+    # the Student-t block sums: n <= W*H <= 512 * 1024 -> at most 512 blocks
+    out["bsum"] = ((W * H + 1023) // 1024, 512)
+    return out
+
+
 def _check_kernels(f, maxes, max_w, max_h, pcs=(0, 5120, 5632, 8192)):
     maxW, maxH, maxN = (int(x) for x in maxes)
     for p in {tuple(int(v) for v in r[:4]) for r in f}:
@@ -198,6 +238,12 @@ def _check_kernels(f, maxes, max_w, max_h, pcs=(0, 5120, 5632, 8192)):
             for layer in range(1, L):
                 assert tr[f"img_{layer}"] < bufsz[f"gray_{layer}"]
             assert tr["img_0"] < W * H
+        if f[0, 21] >= 0:
+            g = _geometric(W, H, L)
+            assert g["depth"] < W * H and g["cloud0"] < W * H
+            for k, v in g.items():
+                if isinstance(v, tuple):
+                    assert v[0] <= v[1], (p, k, v)
 
 
 def _mixed_256():
@@ -206,15 +252,17 @@ def _mixed_256():
     return [SIZES[k] for k in rng.integers(0, 4, 256)]
 
 
-@pytest.mark.parametrize("name,sizes,N", [
-    ("mixed 256 pairs", _mixed_256(), 8192),
-    ("640x480 x 4096", [(640, 480)] * 4096, 8192),
-    ("mixed 8 pairs", SIZES * 2, 2048),
-    ("mixed 6 pairs 12288", [(320, 240), (480, 360), (640, 480)] * 2, 12288),
+@pytest.mark.parametrize("name,sizes,N,mode", [
+    ("mixed 256 pairs", _mixed_256(), 8192, 1),
+    ("640x480 x 4096", [(640, 480)] * 4096, 8192, 1),
+    ("mixed 8 pairs", SIZES * 2, 2048, 1),
+    ("mixed 6 pairs 12288", [(320, 240), (480, 360), (640, 480)] * 2, 12288, 1),
+    ("mixed 64 pairs bi-objective", SIZES * 16, 8192, 2),
+    ("mixed 8 pairs geometric", SIZES * 2, 2048, 0),
 ])
-def test_batch_launches_stay_in_bounds(name, sizes, N):
-    f, order, maxes = _layout(sizes, N)
-    _check_layout(f, order, maxes, 640, 480, N)
+def test_batch_launches_stay_in_bounds(name, sizes, N, mode):
+    f, order, maxes = _layout(sizes, N, mode=mode)
+    _check_layout(f, order, maxes, 640, 480, N, icp=mode != 1)
     _check_kernels(f, maxes, 640, 480)
 
 
@@ -231,7 +279,7 @@ def test_layout_rejects_what_the_kernels_cannot_take():
             arr[i].calib = cal.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         order = np.zeros(n, np.int32)
         maxes = np.zeros(3, np.int32)
-        prm = default_params(n_select=N)
+        prm = default_params(n_select=N, mode=1)
         rc = lib().rt_batch_layout(ctypes.byref(prm), 640, 480, n, arr, f.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)),
                                    order.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
                                    maxes.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))

@@ -30,12 +30,12 @@ TOL_RAD, TOL_MM = 1e-5, 1e-2
 def _variants():
     """{(rz, with point cache): first variant index of that family covering N points}"""
     fam = {}
-    info = (ctypes.c_int * 5)()
+    info = (ctypes.c_int * 6)()
     v = 0
     while lib().rt_batch_variant_info(v, info) == 0:
-        bs, ppt, rz, pc, _ = list(info)
+        bs, ppt, rz, pc, _, mode = list(info)
         covers = rz or bs * ppt >= N
-        if covers and (rz, pc > 0) not in fam:
+        if mode == 1 and covers and (rz, pc > 0) not in fam:
             fam[(rz, pc > 0)] = v
         v += 1
     return fam

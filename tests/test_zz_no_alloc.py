@@ -50,11 +50,11 @@ def _noalloc(pairs, params):
 
 
 def _fixed_variant():
-    info = (ctypes.c_int * 5)()
+    info = (ctypes.c_int * 6)()
     v = 0
     while lib().rt_batch_variant_info(v, info) == 0:
-        bs, ppt, rz, pc, _ = list(info)
-        if bs == 1024 and rz == 1 and pc > 0 and ppt == 4:
+        bs, ppt, rz, pc, _, mode = list(info)
+        if mode == 1 and bs == 1024 and rz == 1 and pc > 0 and ppt == 4:
             return v
         v += 1
     raise AssertionError("no 1024-thread variant with LDS residuals and point cache")
