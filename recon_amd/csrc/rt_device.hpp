@@ -344,5 +344,42 @@ __device__ __forceinline__ float wave_sum32(const float* v) {
     }
     return a[0] + __shfl_xor(a[0], 1, 64);
 }
+// The same for 32 FP64 values: each exchange moves the two halves of a double
+// through the same lane swap.
+__device__ __forceinline__ double swap_add32d(double x, double y) {
+    const unsigned long long xb = __double_as_longlong(x), yb = __double_as_longlong(y);
+    auto lo = __builtin_amdgcn_permlane32_swap((unsigned)xb, (unsigned)yb, false, false);
+    auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(xb >> 32), (unsigned)(yb >> 32), false, false);
+    const double a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    const double b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+    return a + b;
+}
+__device__ __forceinline__ double swap_add16d(double x, double y) {
+    const unsigned long long xb = __double_as_longlong(x), yb = __double_as_longlong(y);
+    auto lo = __builtin_amdgcn_permlane16_swap((unsigned)xb, (unsigned)yb, false, false);
+    auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(xb >> 32), (unsigned)(yb >> 32), false, false);
+    const double a = __longlong_as_double((long long)(((unsigned long long)hi[0] << 32) | lo[0]));
+    const double b = __longlong_as_double((long long)(((unsigned long long)hi[1] << 32) | lo[1]));
+    return a + b;
+}
+__device__ __forceinline__ double wave_sum32d(const double* v) {
+    const int lane = lane_id();
+    double a[16];
+#pragma unroll
+    for (int j = 0; j < 16; j++) a[j] = swap_add32d(v[j], v[16 + j]);
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = swap_add16d(a[j], a[8 + j]);
+#pragma unroll
+    for (int n = 4; n >= 1; n >>= 1) {
+        const bool up = lane & (2 * n);
+#pragma unroll
+        for (int j = 0; j < n; j++) {
+            const double x = a[j], y = a[n + j];
+            const double sx = x + __shfl_xor(x, 2 * n, 64), sy = y + __shfl_xor(y, 2 * n, 64);
+            a[j] = up ? sy : sx;
+        }
+    }
+    return a[0] + __shfl_xor(a[0], 1, 64);
+}
 
 }  // namespace rt

@@ -6,11 +6,12 @@
 // trim rule, ordered scatter), and one gather pass that writes each selected
 // point's position and its per-level Jacobian records.
 // Gauss-Newton iteration: a residual pass (bilinear warp, zWeight, MAD
-// histogram in LDS) and a system pass (robust weights and the 27 unique
-// normal-equation terms: FP32 per workgroup, combined across workgroups in FP64
-// by the last workgroup to finish, which also runs the FP64 solve and the SE(3)
-// update). The geometric side adds an ordered compaction of the valid ICP
-// pixels, three Student-t variance rounds and its own system pass.
+// histogram in LDS), a system pass (robust weights and the 27 unique
+// normal-equation terms: FP32 products summed in FP64 per workgroup) and a
+// one-workgroup pass that combines the workgroups' sums in FP64 and runs the
+// FP64 solve and the SE(3) update. The geometric side adds an ordered
+// compaction of the valid ICP pixels, three Student-t variance rounds and its
+// own system pass, which a bi-objective iteration runs beside the residuals.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "rt_device.hpp"
@@ -18,25 +19,6 @@
 #include "rt_prep.hpp"
 
 namespace rt {
-
-// ------------------------------------------------------------------ tickets
-enum { TK_PHOTO = 0, TK_ICP = 1, TK_ST = 2, TK_FUSE = 3 };
-
-__device__ __forceinline__ bool last_group(unsigned* ticket) {
-    __shared__ bool is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned t = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == gridDim.x * gridDim.y - 1);
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
-}
-__device__ __forceinline__ void reset_ticket(unsigned* ticket) {
-    __hip_atomic_store(ticket, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 
 // ------------------------------------------------------------------ A1 + A2
 // rgbTex2GrayHdrZKernel (rgb2gray.cu:64-76) / u8-u16 conversion, fused with
@@ -160,64 +142,48 @@ void launch_prepare_photo_reference(const Dev& d, hipStream_t s) {
                                                     d.jac[1], d.jac[2], d.jac[3]);
 }
 
-// Partial sums of the normal equations: FP32 per workgroup, term-major
+// Partial sums of the normal equations: FP64 per workgroup, term-major
 // (part[k * PART_STRIDE + group]) so that the combining workgroup reads each
-// term's partials contiguously.
-constexpr int PART_STRIDE = 1024;  // >= the largest grid of a system pass
+// term's partials contiguously. The system passes run at most SYS_GROUPS
+// workgroups of SYS_THREADS: the combine reads partials that other XCDs have
+// just written, and its time grows with their number of cache lines.
 
-// Reduce the 27 per-thread FP32 sums of a 256-thread workgroup (per wave by
-// halving exchanges, then the 4 waves in order) into part[k * PART_STRIDE + group].
-__device__ __forceinline__ void reduce27(const float* acc, float* part) {
-    __shared__ float wsum[4][27];
+// Reduce the 27 per-thread FP64 sums of an NT-thread workgroup (per wave by
+// halving exchanges, then the waves in order) into part[k * stride + group].
+// The per-point products are FP32, their sums FP64: the FP32 oracle's
+// accumulation, so that summation order leaves no trace at FP32 resolution.
+template <int NT>
+__device__ __forceinline__ void reduce27(const double* acc, double* part, int group, int stride = PART_STRIDE) {
+    constexpr int NW = NT / 64;
+    __shared__ double wsum[NW][27];
     const int wave = threadIdx.x >> 6, lane = lane_id();
-    float v32[32];
+    double v32[32];
 #pragma unroll
-    for (int k = 0; k < 32; k++) v32[k] = k < 27 ? acc[k] : 0.0f;
-    const float tot = wave_sum32(v32);
+    for (int k = 0; k < 32; k++) v32[k] = k < 27 ? acc[k] : 0.0;
+    const double tot = wave_sum32d(v32);
     if ((lane & 1) == 0 && (lane >> 1) < 27) wsum[wave][lane >> 1] = tot;
     __syncthreads();
     if (threadIdx.x < 27) {
         const int k = threadIdx.x;
-        part[(size_t)k * PART_STRIDE + blockIdx.x] = ((wsum[0][k] + wsum[1][k]) + wsum[2][k]) + wsum[3][k];
+        double sum = wsum[0][k];
+#pragma unroll
+        for (int w = 1; w < NW; w++) sum += wsum[w][k];
+        part[(size_t)k * stride + group] = sum;
     }
 }
-// Last workgroup (256 threads): the FP64 sum over groups of every term. Wave w
-// takes terms w, w + 4, ...; lane l sums groups l, l + 64, ... in order, then
-// the lanes' sums are added by a fixed exchange tree (deterministic).
-__device__ __forceinline__ void combine27(const float* part, int groups, double* out) {
-    const int wave = threadIdx.x >> 6, lane = lane_id();
-    constexpr int KW = 7;  // terms per wave (4 x 7 >= 27)
-    double s[KW];
-#pragma unroll
-    for (int j = 0; j < KW; j++) s[j] = 0.0;
-    for (int g = lane; g < groups; g += 64) {
-#pragma unroll
-        for (int j = 0; j < KW; j++) {
-            const int k = wave + 4 * j;
-            if (k < 27) s[j] += (double)part[(size_t)k * PART_STRIDE + g];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < KW; j++) {
-        double v = s[j];
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-        const int k = wave + 4 * j;
-        if (lane == 0 && k < 27) out[k] = v;
-    }
-}
-
 // ------------------------------------------------------------------ A10 + A11
 // interpolateResidualKernel2 (warp.cu:269-332) with the level mapping of
 // interpolateResidual2 (:1160-1218); the MAD histogram of discretizeResidual
 // (histogram64.cu:41-44) is built in LDS.
-__global__ __launch_bounds__(256) void k_photo_res(const float4* __restrict__ pts, const float4* __restrict__ jac,
-                                                   int N, int layer, const float* __restrict__ img, int wl, int hl,
-                                                   const float* __restrict__ zcur, int W, int H,
-                                                   const float* __restrict__ c, const float* __restrict__ trel,
-                                                   int bins, float2* __restrict__ res, unsigned* __restrict__ madhist) {
+template <int NT>
+__device__ __forceinline__ void photo_res_body(const float4* __restrict__ pts, const float4* __restrict__ jac, int N,
+                                               int layer, const float* __restrict__ img, int wl, int hl,
+                                               const float* __restrict__ zcur, int W, int H,
+                                               const float* __restrict__ c, const float* __restrict__ trel, int bins,
+                                               float2* __restrict__ res, unsigned* __restrict__ madhist, int grp,
+                                               int ngrp) {
     __shared__ unsigned h[256];
-    h[threadIdx.x] = 0u;
+    if (threadIdx.x < 256) h[threadIdx.x] = 0u;
     float T[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) T[k] = trel[k];
@@ -225,7 +191,7 @@ __global__ __launch_bounds__(256) void k_photo_res(const float4* __restrict__ pt
     float a = 1.0f / (float)(1 << layer);
     float b = 0.5f * (a - 1.0f);
     JacView jv = jac_view(const_cast<float4*>(jac), N);
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < N; i += gridDim.x * 256) {
+    for (int i = grp * NT + threadIdx.x; i < N; i += ngrp * NT) {
         float ir, zW;
         photo_point(pts[i], jv.iref[i], T, a, b, F32Src{img}, wl, hl, F32Src{zcur}, W, H, c, ir, zW);
         res[i] = make_float2(ir, zW);
@@ -238,33 +204,71 @@ __global__ __launch_bounds__(256) void k_photo_res(const float4* __restrict__ pt
         if (v) atomicAdd(&madhist[threadIdx.x], v);
     }
 }
+__global__ __launch_bounds__(256) void k_photo_res(const float4* __restrict__ pts, const float4* __restrict__ jac,
+                                                   int N, int layer, const float* __restrict__ img, int wl, int hl,
+                                                   const float* __restrict__ zcur, int W, int H,
+                                                   const float* __restrict__ c, const float* __restrict__ trel,
+                                                   int bins, float2* __restrict__ res, unsigned* __restrict__ madhist) {
+    photo_res_body<256>(pts, jac, N, layer, img, wl, hl, zcur, W, H, c, trel, bins, res, madhist, blockIdx.x,
+                        gridDim.x);
+}
+
+// The MAD histogram's median bin (generateWeights64Kernel, histogram64.cu:79-94):
+// the first bin whose running count exceeds N / 2 (0 if none), by one wave:
+// lane l holds bins 4l .. 4l+3, a shuffle scan gives the running counts.
+__device__ __forceinline__ void med_bin(const unsigned* __restrict__ madhist, int bins, int N, int* med) {
+    const int lane = threadIdx.x;
+    const uint4 h4 = reinterpret_cast<const uint4*>(madhist)[lane];
+    const int b0 = 4 * lane;
+    const int c0 = b0 < bins ? (int)h4.x : 0, c1 = b0 + 1 < bins ? (int)h4.y : 0;
+    const int c2 = b0 + 2 < bins ? (int)h4.z : 0, c3 = b0 + 3 < bins ? (int)h4.w : 0;
+    const int tot = c0 + c1 + c2 + c3;
+    int inc = tot;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += v;
+    }
+    const int half = N / 2;
+    int r = inc - tot;
+    int first = 256;
+    r += c0; if (first == 256 && r > half) first = b0;
+    r += c1; if (first == 256 && r > half) first = b0 + 1;
+    r += c2; if (first == 256 && r > half) first = b0 + 2;
+    r += c3; if (first == 256 && r > half) first = b0 + 3;
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) first = min(first, __shfl_xor(first, o, 64));
+    if (lane == 0) *med = first < bins ? first : 0;
+}
+// Raw geometric partials (nraw groups, stride RAW_STRIDE) folded into nf
+// groups: folded group f is the sum of raw groups f, f + nf, ... in order.
+// One thread per term.
+__device__ __forceinline__ void fold_partials(const double* __restrict__ raw, int nraw, double* __restrict__ dst, int nf,
+                                              int f, int k) {
+    if (f < nf && k < 27) {
+        double sum = 0.0;
+        for (int g = f; g < nraw; g += nf) sum += raw[(size_t)k * RAW_STRIDE + g];
+        dst[(size_t)k * PART_STRIDE + f] = sum;
+    }
+}
 
 // generateWeights64Kernel (histogram64.cu:79-94) + weightJacobianKernel
-// (warp.cu:770-780) + the 27 dot products (warp.cu:1453-1487).
-__global__ __launch_bounds__(256) void k_photo_sys(const float2* __restrict__ res, const float4* __restrict__ jac,
-                                                   int N, int bins, int robust, unsigned* madhist,
-                                                   float* __restrict__ wts, float* part, double* sys, float* trel,
-                                                   unsigned* ticket, SolveCfg cfg, int solve_here) {
-    __shared__ int hs[256];
+// (warp.cu:770-780) + the 27 dot products (warp.cu:1453-1487): per workgroup
+// FP64 sums of the FP32 products (term-major in part); k_solve combines them.
+template <int NT>
+__device__ __forceinline__ void photo_sys_body(const float2* __restrict__ res, const float4* __restrict__ jac, int N,
+                                               int bins, int robust, const unsigned* __restrict__ madhist,
+                                               float* __restrict__ wts, double* part, int grp, int ngrp) {
     __shared__ int med;
-    int t = threadIdx.x;
-    hs[t] = t < bins ? (int)madhist[t] : 0;
-    if (t == 0) med = 0;
-    __syncthreads();
-    for (int off = 1; off < 256; off <<= 1) {
-        int v = t >= off ? hs[t - off] : 0;
-        __syncthreads();
-        hs[t] += v;
-        __syncthreads();
-    }
-    if (t < bins && hs[t] > N / 2 && (t == 0 || hs[t - 1] <= N / 2)) med = t;
+    const int t = threadIdx.x;
+    if (t < 64) med_bin(madhist, bins, N, &med);
     __syncthreads();
     float s = mad_scale(med, bins);
-    float acc[27];
+    double acc[27];
 #pragma unroll
-    for (int k = 0; k < 27; k++) acc[k] = 0.0f;
+    for (int k = 0; k < 27; k++) acc[k] = 0.0;
     JacView jv = jac_view(const_cast<float4*>(jac), N);
-    for (int i = blockIdx.x * 256 + t; i < N; i += gridDim.x * 256) {
+    for (int i = grp * NT + t; i < N; i += ngrp * NT) {
         float2 rz = res[i];
         float r = rz.x;
         float w = robust_weight(r, rz.y, s, robust);
@@ -277,35 +281,118 @@ __global__ __launch_bounds__(256) void k_photo_sys(const float2* __restrict__ re
 #pragma unroll
         for (int r0 = 0; r0 < 6; r0++)
 #pragma unroll
-            for (int c0 = r0; c0 < 6; c0++) acc[k++] += jw[r0] * jw[c0];
+            for (int c0 = r0; c0 < 6; c0++) acc[k++] += (double)(jw[r0] * jw[c0]);
 #pragma unroll
-        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += jw[r0] * wr;
+        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += (double)(jw[r0] * wr);
     }
-    reduce27(acc, part);
-    if (last_group(ticket)) {
-        combine27(part, gridDim.x, sys);
-        __threadfence_block();
-        __syncthreads();
-        __shared__ double ws[SOLVE_WS];
-        if ((t >> 6) == 0 && solve_here) solve_update_wave(sys, trel, cfg, N, 1, ws);
-        // keep the histogram of this iteration (inspection) and clear it for the next
-        madhist[256 + t] = madhist[t];
+    reduce27<NT>(acc, part, grp);
+}
+// The photometric system over gs workgroups; workgroups below nf also fold the
+// geometric partials of the same iteration (wave 1, beside the median).
+template <int NT>
+__global__ __launch_bounds__(NT) void k_photo_sys(const float2* __restrict__ res, const float4* __restrict__ jac,
+                                                  int N, int bins, int robust, const unsigned* __restrict__ madhist,
+                                                  float* __restrict__ wts, double* part, int gs,
+                                                  const double* __restrict__ raw, int nraw, double* fold, int nf) {
+    if ((threadIdx.x >> 6) == 1) fold_partials(raw, nraw, fold, nf, blockIdx.x, threadIdx.x - 64);
+    if ((int)blockIdx.x < gs) photo_sys_body<NT>(res, jac, N, bins, robust, madhist, wts, part, blockIdx.x, gs);
+}
+
+// The end of an iteration, one workgroup of 512 threads: the photometric
+// partials (gp groups) into sys[0..27) and the geometric ones (gi groups) into
+// sys[27..54) in FP64, the MAD histogram kept for inspection and cleared for
+// the next iteration (madhist non-null), then, when solve is set, the FP64
+// solve and the pose update on one wave (solve_update_wave) from an LDS copy
+// of the system. A separate launch instead of the last workgroup of the system
+// pass: electing that workgroup needs a device-scope fence in every workgroup,
+// which writes back the L2.
+// Wave w sums terms w, w + 8, ..., w + 48 of the 54; lane l adds groups
+// l, l + 64, ... in order (all its loads issued first), then the lanes' sums
+// meet in a fixed exchange tree: the same order for every call. No update
+// from an empty geometric selection (nptr non-null with *nptr = 0).
+constexpr int SOLVE_THREADS = 512;  // 8 waves, 2 per SIMD: room for the FP64 solve
+constexpr int SOLVE_TERMS = 7;     // terms per wave (8 x 7 >= 54)
+constexpr int COMBINE_LOADS = (PART_STRIDE + 63) / 64;  // groups per lane
+__global__ __launch_bounds__(SOLVE_THREADS) void k_solve(const double* __restrict__ part, int gp,
+                                                         const double* __restrict__ part2, int gi, unsigned* madhist,
+                                                         const int* __restrict__ nptr, double* sys, float* trel,
+                                                         SolveCfg cfg, int solve, int N1) {
+    __shared__ double ws[SOLVE_WS];
+    __shared__ double lsys[56];
+    const int t = threadIdx.x, wave = t >> 6, lane = lane_id();
+    const int n2 = nptr ? *nptr : 1;
+    double v[SOLVE_TERMS][COMBINE_LOADS];
+#pragma unroll
+    for (int j = 0; j < SOLVE_TERMS; j++) {
+        const int k = wave + 8 * j;
+        const bool geo = k >= 27;
+        const int groups = k >= 54 ? 0 : (geo ? gi : gp);
+        const double* src = (geo ? part2 : part) + (size_t)(geo ? k - 27 : k) * PART_STRIDE;
+#pragma unroll
+        for (int i = 0; i < COMBINE_LOADS; i++) {
+            const int g = lane + 64 * i;
+            v[j][i] = g < groups ? src[g] : 0.0;
+        }
+    }
+    unsigned mh = 0u;
+    if (madhist && t < 256) mh = madhist[t];
+#pragma unroll
+    for (int j = 0; j < SOLVE_TERMS; j++) {
+        const int k = wave + 8 * j;
+        double acc = 0.0;
+#pragma unroll
+        for (int i = 0; i < COMBINE_LOADS; i++) acc += v[j][i];
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) acc += __shfl_xor(acc, o, 64);
+        const int groups = k >= 54 ? 0 : (k >= 27 ? gi : gp);
+        if (lane == 0 && k < 54) {
+            lsys[k] = acc;
+            if (groups > 0) sys[k] = acc;
+        }
+    }
+    if (madhist && t < 256) {
+        madhist[256 + t] = mh;
         madhist[t] = 0u;
-        if (t == 0) reset_ticket(ticket);
     }
+    __syncthreads();
+    if (solve && wave == 0 && n2 > 0) solve_update_wave(lsys, trel, cfg, N1, n2, ws);
 }
 
 int photo_groups(int N) {
     int g = N / 256;
     return g < 1 ? 1 : (g > 512 ? 512 : g);
 }
+// Workgroups of a system pass over n items (SYS_THREADS each, at most SYS_GROUPS).
+static int sys_groups(int n) {
+    int g = (n + SYS_THREADS - 1) / SYS_THREADS;
+    return g < 1 ? 1 : (g > SYS_GROUPS ? SYS_GROUPS : g);
+}
+// The photometric system pass: 256-thread workgroups while SYS_GROUPS of them
+// cover the points (one point per thread), else SYS_THREADS. Also folds nraw
+// raw geometric partials into nf groups. Returns its number of groups.
+static int launch_photo_sys(const Dev& d, int layer, const SolveCfg& c, const double* raw, int nraw, double* fold,
+                            int nf, hipStream_t s) {
+    const int g256 = (d.N + 255) / 256;
+    if (g256 <= SYS_GROUPS) {
+        const int gs = g256 < 1 ? 1 : g256;
+        k_photo_sys<256><<<gs > nf ? gs : nf, 256, 0, s>>>(d.res, d.jac[layer], d.N, c.mad_bins, c.robust, d.madhist,
+                                                           d.wts, d.part, gs, raw, nraw, fold, nf);
+        return gs;
+    }
+    const int gs = sys_groups(d.N);
+    k_photo_sys<SYS_THREADS><<<gs > nf ? gs : nf, SYS_THREADS, 0, s>>>(d.res, d.jac[layer], d.N, c.mad_bins, c.robust,
+                                                                       d.madhist, d.wts, d.part, gs, raw, nraw, fold,
+                                                                       nf);
+    return gs;
+}
 
-void launch_photo_iteration(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s) {
+void launch_photo_iteration(const Dev& d, int layer, const SolveCfg& c, int finish, hipStream_t s) {
     int G = photo_groups(d.N);
     k_photo_res<<<G, 256, 0, s>>>(d.pts, d.jac[layer], d.N, layer, d.cur.gray[layer], d.W >> layer, d.H >> layer,
                                   d.cur.z, d.W, d.H, d.calib, d.trel, c.mad_bins, d.res, d.madhist);
-    k_photo_sys<<<G, 256, 0, s>>>(d.res, d.jac[layer], d.N, c.mad_bins, c.robust, d.madhist, d.wts, d.part, d.sys,
-                                  d.trel, d.tickets + TK_PHOTO, c, solve_here ? 1 : 0);
+    const int Gs = launch_photo_sys(d, layer, c, nullptr, 0, nullptr, 0, s);
+    k_solve<<<1, SOLVE_THREADS, 0, s>>>(d.part, Gs, nullptr, 0, d.madhist, nullptr, d.sys, d.trel, c,
+                                        finish == FINISH_SOLVE ? 1 : 0, d.N);
 }
 
 // ------------------------------------------------------------------ A18 + A19
@@ -503,15 +590,42 @@ __global__ __launch_bounds__(1024) void k_icp_compact(const float* __restrict__ 
 // generateStudentTWeights (rgb2gray.cu:470-568, 854-877), round k in 1..3: the
 // weights of the previous round are recomputed from its variance, the products
 // are truncated to int per element (one element per thread) and summed by the
-// reference's 1024-wide tree; the last block combines the block sums.
+// reference's 1024-wide tree into this round's block sums. The variance of the
+// previous round, the reference's last-block combination of its block sums
+// (int truncation per block sum, the same tree, clamped to at least n with
+// more than one block), is recomputed by every block from those sums, and
+// stored by block 0; k_student_finish combines the third round's.
+__device__ __forceinline__ void tree1024(float* p, int t) {
+    for (int s = 512; s >= 1; s >>= 1) {
+        if (t < s) p[t] += p[t + s];
+        __syncthreads();
+    }
+}
+__device__ __forceinline__ float student_total(const float* __restrict__ bsum, int nb, int n, float* p, int t) {
+    if (nb == 1) return bsum[0];
+    int ss = 0;
+    for (int q = t; q < nb; q += 1024) ss += bsum[q];
+    p[t] = (float)ss;
+    __syncthreads();
+    tree1024(p, t);
+    float v = p[0];
+    const float mn = (float)n;
+    if (v < mn) v = mn;
+    __syncthreads();
+    return v;
+}
 __global__ __launch_bounds__(1024) void k_student_round(const float* __restrict__ r2, const int* __restrict__ nptr,
-                                                        int round, float* var, float* bsum, unsigned* ticket) {
+                                                        int round, float* var, float* bsum, int stride) {
     __shared__ float p[1024];
-    int n = *nptr;
-    int t = threadIdx.x;
-    int nb = (n + 1023) / 1024;
-    int i = blockIdx.x * 1024 + t;
-    float prev = round > 1 ? var[round - 2] : 0.0f;
+    const int n = *nptr;
+    const int t = threadIdx.x;
+    const int nb = (n + 1023) / 1024;
+    float prev = 0.0f;
+    if (round > 1 && nb >= 1) {
+        prev = student_total(bsum + (size_t)(round - 2) * stride, nb, n, p, t);
+        if (blockIdx.x == 0 && t == 0) var[round - 2] = prev;
+    }
+    const int i = blockIdx.x * 1024 + t;
     int sum = 0;
     if (i < n) {
         float x = r2[i];
@@ -520,79 +634,91 @@ __global__ __launch_bounds__(1024) void k_student_round(const float* __restrict_
     }
     p[t] = (float)sum;
     __syncthreads();
-    for (int s = 512; s >= 1; s >>= 1) {
-        if (t < s) p[t] += p[t + s];
-        __syncthreads();
-    }
-    if (t == 0) bsum[blockIdx.x] = p[0];
-    if (last_group(ticket)) {
-        if (nb >= 1) {
-            if (nb == 1) {
-                if (t == 0) var[round - 1] = bsum[0];
-            } else {
-                int ss = 0;
-                for (int q = t; q < nb; q += 1024) ss += bsum[q];
-                p[t] = (float)ss;
-                __syncthreads();
-                for (int s = 512; s >= 1; s >>= 1) {
-                    if (t < s) p[t] += p[t + s];
-                    __syncthreads();
-                }
-                if (t == 0) {
-                    float v = p[0];
-                    float mn = (float)n;
-                    if (v < mn) v = mn;
-                    var[round - 1] = v;
-                }
-            }
-        }
-        if (t == 0) reset_ticket(ticket);
-    }
+    tree1024(p, t);
+    if (t == 0) bsum[(size_t)(round - 1) * stride + blockIdx.x] = p[0];
+}
+__global__ __launch_bounds__(1024) void k_student_finish(const int* __restrict__ nptr, float* var,
+                                                         const float* __restrict__ bsum, int stride) {
+    __shared__ float p[1024];
+    const int n = *nptr;
+    const int nb = (n + 1023) / 1024;
+    if (nb < 1) return;
+    const float v = student_total(bsum + 2 * (size_t)stride, nb, n, p, threadIdx.x);
+    if (threadIdx.x == 0) var[2] = v;
 }
 
 // icpResidualIndexKernel2 (rgb2gray.cu:323-390) with the final clamped
-// Student-t weight, and the 27 dot products of its system.
-__global__ __launch_bounds__(256) void k_icp_sys(const float4* __restrict__ rp, const float4* __restrict__ rn,
-                                                 const int* __restrict__ sel, const float* __restrict__ r2,
-                                                 const int* __restrict__ nptr, const float* __restrict__ var, int w,
-                                                 int h, int layer, const float* __restrict__ c,
-                                                 const float4* __restrict__ cp, float* part, double* sys, float* trel,
-                                                 unsigned* ticket, SolveCfg cfg, int solve_here, int N1) {
+// Student-t weight, and the 27 dot products of its system: per workgroup FP64
+// sums of the FP32 products (term-major in part); k_solve combines them.
+struct IcpSysArgs {
+    const float4* rp;
+    const float4* rn;
+    const int* sel;
+    const float* r2;
+    const int* nptr;
+    const float* var;
+    int w, h, layer;
+    const float* c;
+    const float4* cp;
+    const float* trel;
+    double* part;
+    int stride;  // of part
+    float scale_in;
+};
+template <int NT>
+__device__ __forceinline__ void icp_sys_body(const IcpSysArgs& g, int grp, int ngrp) {
+    const float4* __restrict__ rp = g.rp;
+    const float4* __restrict__ rn = g.rn;
+    const int* __restrict__ sel = g.sel;
+    const float* __restrict__ r2 = g.r2;
+    const float* __restrict__ c = g.c;
+    const float4* __restrict__ cp = g.cp;
+    const float* __restrict__ trel = g.trel;
+    const float* __restrict__ var = g.var;
+    const int w = g.w, h = g.h, layer = g.layer;
+    const float scale_in = g.scale_in;
+    const int* nptr = g.nptr;
     int n = *nptr;
     float T[16];
 #pragma unroll
     for (int k = 0; k < 16; k++) T[k] = trel[k];
-    float acc[27];
+    double acc[27];
 #pragma unroll
-    for (int k = 0; k < 27; k++) acc[k] = 0.0f;
+    for (int k = 0; k < 27; k++) acc[k] = 0.0;
     float v3 = var[2];
     float a = 1.0f / (float)(1 << layer), b = 0.5f * (a - 1.0f);
-    float s = cfg.scale_in;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+    for (int i = grp * NT + threadIdx.x; i < n; i += ngrp * NT) {
         float res, J[6];
-        icp_point_terms(rp, rn, cp, w, h, sel[i], r2[i], v3, n, T, c, a, b, s, res, J);
+        icp_point_terms(rp, rn, cp, w, h, sel[i], r2[i], v3, n, T, c, a, b, scale_in, res, J);
         int k = 0;
 #pragma unroll
         for (int r0 = 0; r0 < 6; r0++)
 #pragma unroll
-            for (int c0 = r0; c0 < 6; c0++) acc[k++] += J[r0] * J[c0];
+            for (int c0 = r0; c0 < 6; c0++) acc[k++] += (double)(J[r0] * J[c0]);
 #pragma unroll
-        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += J[r0] * res;
+        for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += (double)(J[r0] * res);
     }
-    reduce27(acc, part);
-    if (last_group(ticket)) {
-        combine27(part, gridDim.x, sys + 27);
-        __threadfence_block();
-        __syncthreads();
-        __shared__ double ws[SOLVE_WS];
-        if ((threadIdx.x >> 6) == 0 && solve_here && n > 0) solve_update_wave(sys, trel, cfg, N1, n, ws);
-        if (threadIdx.x == 0) reset_ticket(ticket);
-    }
+    reduce27<NT>(acc, g.part, grp, g.stride);
 }
-
-int icp_groups(int npix) {
-    int g = (npix + 255) / 256;
-    return g < 1 ? 1 : (g > 1024 ? 1024 : g);
+__global__ __launch_bounds__(SYS_THREADS) void k_icp_sys(IcpSysArgs g) {
+    icp_sys_body<SYS_THREADS>(g, blockIdx.x, gridDim.x);
+}
+// A bi-objective iteration's photometric residuals (first gp workgroups) and
+// its geometric system (the rest) in one launch: the geometric system does not
+// depend on the residuals, so it leaves the path to the solve; each half works
+// as its own kernel would.
+__global__ __launch_bounds__(SYS_THREADS) void k_bi_res(const float4* __restrict__ pts,
+                                                        const float4* __restrict__ jac, int N, int layer,
+                                                        const float* __restrict__ img, int wl, int hl,
+                                                        const float* __restrict__ zcur, int W, int H,
+                                                        const float* __restrict__ c, int bins,
+                                                        float2* __restrict__ res, unsigned* __restrict__ madhist,
+                                                        int gp, IcpSysArgs g) {
+    if ((int)blockIdx.x < gp)
+        photo_res_body<SYS_THREADS>(pts, jac, N, layer, img, wl, hl, zcur, W, H, c, g.trel, bins, res, madhist,
+                                    blockIdx.x, gp);
+    else
+        icp_sys_body<SYS_THREADS>(g, blockIdx.x - gp, gridDim.x - gp);
 }
 
 void launch_icp_select(const Dev& d, int layer, hipStream_t s) {
@@ -603,17 +729,40 @@ void launch_icp_select(const Dev& d, int layer, hipStream_t s) {
                                   words, d.icp_cnt);
     k_icp_compact<<<G, 1024, 0, s>>>(d.rfull, words, d.icp_cnt, G, d.icp_sel[layer], d.icp_r2[layer],
                                      d.icp_n + layer);
+    const int stride = (d.W * d.H + 1023) / 1024;
     for (int r = 1; r <= 3; r++)
-        k_student_round<<<G, 1024, 0, s>>>(d.icp_r2[layer], d.icp_n + layer, r, d.icp_var + layer * 4, d.bsum,
-                                           d.tickets + TK_ST);
+        k_student_round<<<G, 1024, 0, s>>>(d.icp_r2[layer], d.icp_n + layer, r, d.icp_var + layer * 4, d.bsum, stride);
+    k_student_finish<<<1, 1024, 0, s>>>(d.icp_n + layer, d.icp_var + layer * 4, d.bsum, stride);
 }
 
-void launch_icp_system(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s) {
-    int w = d.W >> layer, h = d.H >> layer;
-    int G = icp_groups(w * h);
-    k_icp_sys<<<G, 256, 0, s>>>(d.rpos[layer], d.rnrm[layer], d.icp_sel[layer], d.icp_r2[layer], d.icp_n + layer,
-                                d.icp_var + layer * 4, w, h, layer, d.calib, d.cpos[layer], d.part, d.sys, d.trel,
-                                d.tickets + TK_ICP, c, solve_here ? 1 : 0, d.N);
+static IcpSysArgs icp_sys_args(const Dev& d, int layer, const SolveCfg& c, double* part, int stride) {
+    return IcpSysArgs{d.rpos[layer], d.rnrm[layer], d.icp_sel[layer], d.icp_r2[layer], d.icp_n + layer,
+                      d.icp_var + layer * 4, d.W >> layer, d.H >> layer, layer, d.calib, d.cpos[layer], d.trel,
+                      part, stride, c.scale_in};
+}
+
+void launch_icp_system(const Dev& d, int layer, const SolveCfg& c, int finish, hipStream_t s) {
+    const int Gi = sys_groups((d.W >> layer) * (d.H >> layer));
+    k_icp_sys<<<Gi, SYS_THREADS, 0, s>>>(icp_sys_args(d, layer, c, d.part2, PART_STRIDE));
+    k_solve<<<1, SOLVE_THREADS, 0, s>>>(nullptr, 0, d.part2, Gi, nullptr, d.icp_n + layer, d.sys, d.trel, c,
+                                        finish == FINISH_SOLVE ? 1 : 0, d.N);
+}
+
+// The geometric system runs beside the photometric residuals on up to
+// RAW_STRIDE workgroups (every CU busy); the photometric system pass folds its
+// partials to at most SYS_GROUPS for the combine.
+void launch_biobjective_iteration(const Dev& d, int layer, const SolveCfg& c, int finish, bool select, hipStream_t s) {
+    const int Gr = d.N > 0 ? (d.N + SYS_THREADS - 1) / SYS_THREADS : 1;  // residuals, one point per thread
+    int Gi = ((d.W >> layer) * (d.H >> layer) + SYS_THREADS - 1) / SYS_THREADS;
+    Gi = Gi < 1 ? 1 : (Gi > RAW_STRIDE ? RAW_STRIDE : Gi);
+    const int nf = Gi < SYS_GROUPS ? Gi : SYS_GROUPS;
+    if (select) launch_icp_select(d, layer, s);
+    k_bi_res<<<Gr + Gi, SYS_THREADS, 0, s>>>(d.pts, d.jac[layer], d.N, layer, d.cur.gray[layer], d.W >> layer,
+                                             d.H >> layer, d.cur.z, d.W, d.H, d.calib, c.mad_bins, d.res, d.madhist,
+                                             Gr, icp_sys_args(d, layer, c, d.part3, RAW_STRIDE));
+    const int Gs = launch_photo_sys(d, layer, c, d.part3, Gi, d.part2, nf, s);
+    k_solve<<<1, SOLVE_THREADS, 0, s>>>(d.part, Gs, d.part2, nf, d.madhist, d.icp_n + layer, d.sys, d.trel, c,
+                                        finish == FINISH_SOLVE ? 1 : 0, d.N);
 }
 
 // ------------------------------------------------------------------ A25

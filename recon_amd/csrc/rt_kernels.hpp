@@ -45,10 +45,9 @@ struct Dev {
     float2* res;                // N: r, zWeight
     float* wts;                 // N: robust weight of the last iteration
     unsigned* madhist;          // 512: accumulating histogram | copy of the last one
-    float* part;                // partial sums: max groups * 27
+    double* part;               // photometric system partials (27 x PART_STRIDE)
     double* sys;                // 54: photometric 27 | geometric 27
     float* trel;                // 16
-    unsigned* tickets;          // 16 counters
     // geometric
     float4* rpos[4];            // ref cloud (x,y,z,mask)
     float4* rnrm[4];            // ref normals
@@ -60,7 +59,9 @@ struct Dev {
     float* icp_r2[4];           // per level
     float* icp_var;             // 4 levels x 4 (3 rounds + spare)
     int* icp_n;                 // 4
-    float* bsum;                // Student-t block sums (W*H/1024)
+    float* bsum;                // Student-t block sums, 3 rounds x ceil(W*H/1024)
+    double* part2;              // geometric system partials (27 x PART_STRIDE)
+    double* part3;              // raw geometric partials (27 x RAW_STRIDE)
     double* pose;               // device-side pose state for sequences: Tprev(16 float as double) | global(16)
     // inspection images (updateErrorImage, getDebugImage1C, getRefImage1C)
     float* dbg_err;             // W*H photometric error image
@@ -88,14 +89,28 @@ void launch_ref_cloud(const Dev& d, hipStream_t s);
 void launch_ref_oriented(const Dev& d, const float* xyz, const float* nrm, hipStream_t s);
 void launch_ref_cloud_pyramid(const Dev& d, hipStream_t s);
 void launch_cur_cloud(const Dev& d, hipStream_t s);
-void launch_photo_iteration(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s);
+// How an iteration's system pass ends: combined into sys only (inspection), or
+// combined, solved and the pose updated.
+enum { FINISH_COMBINE = 1, FINISH_SOLVE = 2 };
+// System passes: at most SYS_GROUPS workgroups of SYS_THREADS threads, each
+// leaving 27 FP64 partial sums at part[k * PART_STRIDE + group].
+constexpr int SYS_THREADS = 1024;
+constexpr int SYS_GROUPS = 64;
+constexpr int PART_STRIDE = SYS_GROUPS;
+// Geometric system beside the photometric residuals: up to RAW_STRIDE
+// workgroups, partials at part3[k * RAW_STRIDE + group].
+constexpr int RAW_STRIDE = 256;
+void launch_photo_iteration(const Dev& d, int layer, const SolveCfg& c, int finish, hipStream_t s);
 void launch_icp_select(const Dev& d, int layer, hipStream_t s);
-void launch_icp_system(const Dev& d, int layer, const SolveCfg& c, bool solve_here, hipStream_t s);
+// finish is FINISH_COMBINE or FINISH_SOLVE.
+void launch_icp_system(const Dev& d, int layer, const SolveCfg& c, int finish, hipStream_t s);
+// One bi-objective iteration: photometric residuals, the ICP selection of the
+// level (select), both systems in one launch, the combine (and solve).
+void launch_biobjective_iteration(const Dev& d, int layer, const SolveCfg& c, int finish, bool select, hipStream_t s);
 void launch_fuse(const Dev& d, hipStream_t s);
 void launch_error_image_photo(const Dev& d, hipStream_t s);
 void launch_error_image_icp(const Dev& d, hipStream_t s);
 void launch_apply_increment(const Dev& d, double* pose_out, hipStream_t s);
 int photo_groups(int N);
-int icp_groups(int npix);
 
 }  // namespace rt

@@ -168,13 +168,13 @@ extern "C" int rt_create(int device, int W, int H, const float* calib, const rt_
     A(d.gmag, npx); A(d.ghist, 256); A(d.seg, 2600); A(d.sel, p.n_select); A(d.pts, p.n_select);
     for (int l = 0; l < L; l++) A(d.jac[l], (jac_bytes(p.n_select) + 15) / 16);
     A(d.res, p.n_select); A(d.wts, p.n_select); A(d.madhist, 512);
-    A(d.part, 27 * 1100); A(d.sys, 64); A(d.trel, 16); A(d.tickets, 16);
+    A(d.part, 27 * PART_STRIDE); A(d.part2, 27 * PART_STRIDE); A(d.part3, 27 * RAW_STRIDE); A(d.sys, 64); A(d.trel, 16);
     for (int l = 0; l < L; l++) {
         size_t nl = (size_t)(W >> l) * (H >> l);
         A(d.rpos[l], nl); A(d.rnrm[l], nl); A(d.cpos[l], nl); A(d.icp_sel[l], nl); A(d.icp_r2[l], nl);
     }
     A(d.rfull, npx); A(d.icp_cnt, G1024); A(d.icp_bits, (size_t)G1024 * 16); A(d.icp_var, 16); A(d.icp_n, 4);
-    A(d.bsum, G1024); A(d.pose, 32);
+    A(d.bsum, 3 * (size_t)G1024); A(d.pose, 32);
     A(d.dbg_err, npx); A(d.dbg_ref, npx);
     for (int l = 0; l < L; l++) {
         size_t nl = (size_t)(W >> l) * (H >> l);
@@ -474,14 +474,12 @@ static void estimate_launches(rt_tracker* t, const SolveCfg& c) {
         int n = t->prm.iters[t->L - 1 - layer];
         for (int j = 0; j < n; j++) {
             if (m == RT_PHOTOMETRIC) {
-                launch_photo_iteration(t->d, layer, c, true, t->stream);
+                launch_photo_iteration(t->d, layer, c, FINISH_SOLVE, t->stream);
             } else if (m == RT_GEOMETRIC) {
                 if (j == 0) launch_icp_select(t->d, layer, t->stream);
-                launch_icp_system(t->d, layer, c, true, t->stream);
+                launch_icp_system(t->d, layer, c, FINISH_SOLVE, t->stream);
             } else {
-                launch_photo_iteration(t->d, layer, c, false, t->stream);
-                if (j == 0) launch_icp_select(t->d, layer, t->stream);
-                launch_icp_system(t->d, layer, c, true, t->stream);
+                launch_biobjective_iteration(t->d, layer, c, FINISH_SOLVE, j == 0, t->stream);
             }
         }
     }
@@ -861,12 +859,13 @@ extern "C" int rt_debug_linearize(rt_tracker* t, int level, int it) {
     HIP_TRY(hipSetDevice(t->device));
     SolveCfg c = solve_cfg(t);
     int m = t->mode;
-    if (m == RT_PHOTOMETRIC || m == RT_BIOBJECTIVE) {
-        launch_photo_iteration(t->d, level, c, false, t->stream);
-    }
-    if (m == RT_GEOMETRIC || m == RT_BIOBJECTIVE) {
+    if (m == RT_PHOTOMETRIC) {
+        launch_photo_iteration(t->d, level, c, FINISH_COMBINE, t->stream);
+    } else if (m == RT_GEOMETRIC) {
         if (it == 0) launch_icp_select(t->d, level, t->stream);
-        launch_icp_system(t->d, level, c, false, t->stream);
+        launch_icp_system(t->d, level, c, FINISH_COMBINE, t->stream);
+    } else {
+        launch_biobjective_iteration(t->d, level, c, FINISH_COMBINE, it == 0, t->stream);
     }
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipStreamSynchronize(t->stream));
