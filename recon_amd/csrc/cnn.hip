@@ -459,6 +459,10 @@ __global__ void k_pack(const float* __restrict__ W, int Co, int Ci, int Kp, bf16
 }
 
 // ---------------------------------------------------------------- inputs and proposals
+// The network's inputs, one cell of (W / IW) x (H / IH) pixels per thread and
+// per frame: mean gray / 255 - 0.5 and the mean inverse depth (1000 / mm, by
+// the hardware reciprocal: within 1 ulp, far below the bfloat16 rounding of
+// the result) of the valid pixels - 1, or -1 without valid depth.
 __global__ __launch_bounds__(256) void k_inputs(const FrameSrc* __restrict__ src, int IW, int IH, bf16* __restrict__ x) {
     const int n = blockIdx.y;
     const int cell = blockIdx.x * 256 + threadIdx.x;
@@ -485,13 +489,13 @@ __global__ __launch_bounds__(256) void k_inputs(const FrameSrc* __restrict__ src
                 for (int xx = 0; xx < 8; xx++) {
                     gs += (gw[xx >> 2] >> (8 * (xx & 3))) & 0xffu;
                     unsigned d = (dw[xx >> 1] >> (16 * (xx & 1))) & 0xffffu;
-                    if (d) { ids += 1000.0f / (float)d; nv++; }
+                    if (d) { ids += 1000.0f * __builtin_amdgcn_rcpf((float)d); nv++; }
                 }
             } else {
                 for (int xx = 0; xx < bw; xx++) {
                     gs += g8[row + xx];
                     unsigned d = d16[row + xx];
-                    if (d) { ids += 1000.0f / (float)d; nv++; }
+                    if (d) { ids += 1000.0f * __builtin_amdgcn_rcpf((float)d); nv++; }
                 }
             }
         }

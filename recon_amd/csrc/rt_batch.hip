@@ -899,6 +899,12 @@ struct rt_batch {
     int device = 0, max_pairs = 0, max_w = 0, max_h = 0;
     rt_params prm;
     hipStream_t stream = nullptr;
+    // The pose network's proposals run on a stream of their own beside the
+    // keyframe preparation (they read only the frames and write the initial
+    // poses): it waits for the prepare call's uploads and pyramids (inputs),
+    // and the batch stream waits for the proposals (proposed) before tracking.
+    hipStream_t aux = nullptr;
+    hipEvent_t inputs = nullptr, proposed = nullptr;
     int cus = 256;
     std::vector<void*> allocs;
     // per-pair arenas
@@ -922,7 +928,7 @@ struct rt_batch {
     // tuning and graphs
     std::map<std::string, std::pair<int, int>> tuned;  // key -> (variant, grid)
     int variant = -1, grid = 0;
-    hipGraphExec_t graph_prep = nullptr, graph_track = nullptr;
+    hipGraphExec_t graph_pyr = nullptr, graph_prep = nullptr, graph_track = nullptr;
     int graph_n = -1, graph_variant = -1, graph_grid = -1, graph_maxN = -1;  // the track graph's launch shape
 };
 
@@ -970,12 +976,18 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, device) == hipSuccess) b->cus = prop.multiProcessorCount;
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return set_error(RT_ERR_HIP, "stream"); }
-    if (b->prm.mode < RT_GEOMETRIC || b->prm.mode > RT_BIOBJECTIVE) { delete b; return set_error(RT_ERR_ARG, "bad mode"); }
+    if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&b->inputs, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->proposed, hipEventDisableTiming) != hipSuccess) {
+        rt_batch_destroy(b);
+        return set_error(RT_ERR_HIP, "stream");
+    }
+    if (b->prm.mode < RT_GEOMETRIC || b->prm.mode > RT_BIOBJECTIVE) { rt_batch_destroy(b); return set_error(RT_ERR_ARG, "bad mode"); }
     b->prm.lambda = std::min(1.0f, std::max(0.0f, b->prm.lambda));  // setLambda's clamp (phototracker.cpp:163-168)
     b->pair_bytes = pair_arena_bytes(max_w, max_h, (size_t)b->prm.n_select, b->prm.mode != RT_PHOTOMETRIC);
     size_t total = b->pair_bytes * (size_t)max_pairs;
     void* q = nullptr;
-    if (hipMalloc(&q, total) != hipSuccess) { hipStreamDestroy(b->stream); delete b; return set_error(RT_ERR_NOMEM, "batch arena"); }
+    if (hipMalloc(&q, total) != hipSuccess) { rt_batch_destroy(b); return set_error(RT_ERR_NOMEM, "batch arena"); }
     b->arena = (char*)q; b->allocs.push_back(q);
     auto dal = [&](void** p, size_t bytes) {
         if (hipMalloc(p, bytes) != hipSuccess) return false;
@@ -988,9 +1000,7 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
               dal((void**)&b->trel_in, 64 * (size_t)max_pairs) && dal((void**)&b->trel_out, 64 * (size_t)max_pairs) &&
               dal((void**)&b->rms, 4 * (size_t)max_pairs) && dal((void**)&b->status, 4 * (size_t)max_pairs);
     if (!ok) {
-        for (void* p : b->allocs) hipFree(p);
-        hipStreamDestroy(b->stream);
-        delete b;
+        rt_batch_destroy(b);
         return set_error(RT_ERR_NOMEM, "batch buffers");
     }
     *out = b;
@@ -1000,12 +1010,17 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
 extern "C" void rt_batch_destroy(rt_batch* b) {
     if (!b) return;
     hipSetDevice(b->device);
-    hipStreamSynchronize(b->stream);
+    if (b->aux) hipStreamSynchronize(b->aux);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    if (b->graph_pyr) hipGraphExecDestroy(b->graph_pyr);
     if (b->graph_prep) hipGraphExecDestroy(b->graph_prep);
     if (b->graph_track) hipGraphExecDestroy(b->graph_track);
     for (void* p : b->allocs) hipFree(p);
     b->st_desc.release(); b->st_calib.release(); b->st_order.release(); b->st_trel.release();
-    hipStreamDestroy(b->stream);
+    if (b->inputs) hipEventDestroy(b->inputs);
+    if (b->proposed) hipEventDestroy(b->proposed);
+    if (b->aux) hipStreamDestroy(b->aux);
+    if (b->stream) hipStreamDestroy(b->stream);
     delete b;
 }
 
@@ -1027,11 +1042,16 @@ static BatchView view_of(const rt_batch* b) {
                      b->prm.solver, b->prm.robust, b->prm.mad_bins, b->prm.mode, b->prm.lambda};
 }
 
-static void launch_prep(const BatchView& v, hipStream_t s) {
+// Keyframe preparation in two parts: the pyramids of both frames (part 1) and
+// the gradient bins, selection, points and clouds (part 2); 0: both.
+static void launch_prep(const BatchView& v, hipStream_t s, int part = 0) {
     const int n = v.n;
-    dim3 g64((v.maxW + 63) / 64, (v.maxH + 63) / 64, n);
-    kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 0);
-    kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 1);
+    if (part != 2) {
+        dim3 g64((v.maxW + 63) / 64, (v.maxH + 63) / 64, n);
+        kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 0);
+        kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 1);
+    }
+    if (part == 1) return;
     dim3 gb((v.maxW + 63) / 64, (v.maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
     kb_bins<<<gb, 256, 0, s>>>(v.desc);
     kb_select<<<n, SEL_BS, 0, s>>>(v.desc);
@@ -1167,10 +1187,11 @@ extern "C" int rt_batch_variant_info(int v, int info[6]) {
     return RT_OK;
 }
 
-static int capture(rt_batch* b, hipGraphExec_t* out, bool prep) {
+// prep: 0 the tracking launch, else the part of the preparation (launch_prep)
+static int capture(rt_batch* b, hipGraphExec_t* out, int prep) {
     hipGraph_t g;
     if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return set_error(RT_ERR_HIP, "begin capture");
-    if (prep) launch_prep(view_of(b), b->stream);
+    if (prep) launch_prep(view_of(b), b->stream, prep);
     else launch_track(b, b->variant, b->grid, b->stream);
     if (hipStreamEndCapture(b->stream, &g) != hipSuccess) return set_error(RT_ERR_HIP, "end capture");
     if (*out) hipGraphExecDestroy(*out);
@@ -1208,7 +1229,7 @@ static int run_track(rt_batch* b, rt_pair_result* out) {
     // through its dynamic LDS, on the largest point count
     if (!b->graph_track || b->graph_n != b->n || b->graph_variant != b->variant || b->graph_grid != b->grid ||
         b->graph_maxN != b->maxN) {
-        if ((rc = capture(b, &b->graph_track, false))) return rc;
+        if ((rc = capture(b, &b->graph_track, 0))) return rc;
         b->graph_n = b->n; b->graph_variant = b->variant; b->graph_grid = b->grid; b->graph_maxN = b->maxN;
     }
     if (hipGraphLaunch(b->graph_track, b->stream) != hipSuccess) return set_error(RT_ERR_HIP, "graph launch");
@@ -1305,11 +1326,15 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
         return set_error(RT_ERR_HIP, "descriptor upload");
     bool changed = b->n != n || b->maxW != maxW || b->maxH != maxH || b->maxN != maxN;
     b->n = n; b->desc = desc; b->maxW = maxW; b->maxH = maxH; b->maxN = maxN;
-    if (changed || !b->graph_prep) {
-        rc = capture(b, &b->graph_prep, true);
-        if (rc) return rc;
+    if (changed || !b->graph_pyr || !b->graph_prep) {
+        if ((rc = capture(b, &b->graph_pyr, 1)) || (rc = capture(b, &b->graph_prep, 2))) return rc;
     }
-    if (hipGraphLaunch(b->graph_prep, b->stream) != hipSuccess) return set_error(RT_ERR_HIP, "graph launch");
+    // the pose network's proposals (rt_batch_propose) start after the pyramids:
+    // both read whole frames, while the gradient bins that follow are bound by
+    // arithmetic
+    if (hipGraphLaunch(b->graph_pyr, b->stream) != hipSuccess || hipEventRecord(b->inputs, b->stream) != hipSuccess ||
+        hipGraphLaunch(b->graph_prep, b->stream) != hipSuccess)
+        return set_error(RT_ERR_HIP, "graph launch");
     return RT_OK;
 }
 
@@ -1328,7 +1353,11 @@ extern "C" int rt_batch_propose(rt_batch* b, rt_cnn* net) {
         src[i].g8[0] = d.g8[0]; src[i].g8[1] = d.g8[1];
         src[i].d16[0] = d.d16[0]; src[i].d16[1] = d.d16[1];
     }
-    return cnn_propose(net, b->n, src.data(), b->trel_in, b->stream);
+    if (hipStreamWaitEvent(b->aux, b->inputs, 0) != hipSuccess) return set_error(RT_ERR_HIP, "rt_batch_propose: wait");
+    const int rc = cnn_propose(net, b->n, src.data(), b->trel_in, b->aux);
+    if (hipEventRecord(b->proposed, b->aux) != hipSuccess || hipStreamWaitEvent(b->stream, b->proposed, 0) != hipSuccess)
+        return set_error(RT_ERR_HIP, "rt_batch_propose: join");
+    return rc;
 }
 
 // Inspection of one pair of the last prepared batch: RT_DBG_GRAD_HIST,
