@@ -2,7 +2,9 @@
 // cores (bfloat16 operands, FP32 accumulation), Adam, and the C ABI of
 // include/rt_cnn.h.
 //
-// Every convolution is an implicit GEMM over NHWC bfloat16 tensors, built from
+// Every convolution is an implicit GEMM over NHWC bfloat16 tensors. The
+// forward pass of the default network runs on v_mfma_f32_16x16x32_bf16
+// (k_conv_fwd_mfma); other layer shapes and the backward passes use
 // v_mfma_f32_16x16x16_bf16 tiles (64-lane wave, A: 16 rows x 4 k per lane
 // group, B: 4 k x 16 columns, D: 4 rows x 16 columns per lane group):
 //   forward      rows = output pixels (n, yo, xo), columns = Cout, k = (tap, Cin)
@@ -200,6 +202,123 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(ConvGeom g, const bf16* __
                     Y[(size_t)m * g.Co + col] = f2bf(v > 0.0f ? v : 0.0f);
                 }
             }
+    }
+}
+
+// Forward on gfx950's v_mfma_f32_16x16x32_bf16 (A: row lane % 16, eight
+// consecutive k per lane group lane / 16; B the same k for column lane % 16;
+// D: rows 4 * (lane / 16) + i, column lane % 16). Each wave owns a 16*TM x
+// 16*TN tile; four waves (WGM x WGN) form the workgroup. Operands come
+// straight from global memory as 16-byte loads (an 8-channel run of one tap
+// of one input pixel, or 8 k of one weight row; two taps of 4 channels when
+// Cin = 4), the next k-step's loads issued before the current step's MFMAs.
+// The bias, ReLU and bfloat16 rounding are applied to the accumulators, and
+// the tile leaves through LDS as 16-byte row chunks.
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+__device__ __forceinline__ f4 mfma32(bf8 a, bf8 b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+template <int TM, int TN, int WGM, int WGN, bool CI4>
+__global__ __launch_bounds__(256) void k_conv_fwd_mfma(ConvGeom g, const bf16* __restrict__ X,
+                                                       const bf16* __restrict__ Wp, const float* __restrict__ bias,
+                                                       bf16* __restrict__ Y) {
+    static_assert(WGM * WGN == 4, "four waves");
+    constexpr int WM = 16 * TM, WN = 16 * TN, LDT = WN + 8;  // LDS row pitch (bf16)
+    __shared__ __attribute__((aligned(16))) bf16 tile[4][WM * LDT];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int r16 = lane & 15, kg = lane >> 4;
+    const int M = g.B * g.Ho * g.Wo, HW = g.Ho * g.Wo;
+    const int row0 = blockIdx.x * (WM * WGM) + (w % WGM) * WM;
+    const int col0 = blockIdx.y * (WN * WGN) + (w / WGM) * WN;
+    int yb[TM], xb[TM];
+    size_t nb[TM];
+    bool mv[TM];
+#pragma unroll
+    for (int t = 0; t < TM; t++) {
+        const int m = row0 + 16 * t + r16;
+        mv[t] = m < M;
+        const int mm = mv[t] ? m : 0;
+        const int n = mm / HW, r = mm - n * HW, yo = r / g.Wo, xo = r - yo * g.Wo;
+        yb[t] = 2 * yo - 1;
+        xb[t] = 2 * xo - 1;
+        nb[t] = (size_t)n * g.Hi * g.Wi;
+    }
+    const bf16* wrow[TN];
+#pragma unroll
+    for (int j = 0; j < TN; j++) wrow[j] = Wp + (size_t)(col0 + 16 * j + r16) * g.Kp + 8 * kg;
+    auto tap_ok = [&](int t, int tap, size_t& off) {
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+        const int yi = yb[t] + ky, xi = xb[t] + kx;
+        off = (nb[t] + (size_t)yi * g.Wi + xi) * g.Ci;
+        return mv[t] && tap < 9 && yi >= 0 && yi < g.Hi && xi >= 0 && xi < g.Wi;
+    };
+    auto load = [&](int s, bf8* a, bf8* b) {
+        const int k = 32 * s + 8 * kg;
+        if constexpr (CI4) {
+            const int tap0 = k >> 2;
+#pragma unroll
+            for (int t = 0; t < TM; t++) {
+                size_t o0, o1;
+                const bool ok0 = tap_ok(t, tap0, o0), ok1 = tap_ok(t, tap0 + 1, o1);
+                const uint2 v0 = ok0 ? *(const uint2*)(X + o0) : make_uint2(0u, 0u);
+                const uint2 v1 = ok1 ? *(const uint2*)(X + o1) : make_uint2(0u, 0u);
+                const uint4 v = make_uint4(v0.x, v0.y, v1.x, v1.y);
+                a[t] = __builtin_bit_cast(bf8, v);
+            }
+        } else {
+            const int tap = k >> g.ci_shift, ci = k & (g.Ci - 1);
+#pragma unroll
+            for (int t = 0; t < TM; t++) {
+                size_t o;
+                const bool ok = tap_ok(t, tap, o);
+                const uint4 v = ok ? *(const uint4*)(X + o + ci) : make_uint4(0u, 0u, 0u, 0u);
+                a[t] = __builtin_bit_cast(bf8, v);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < TN; j++) b[j] = __builtin_bit_cast(bf8, *(const uint4*)(wrow[j] + 32 * s));
+    };
+    f4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    const int steps = g.Kp / 32;
+    bf8 a[TM], b[TN];
+    load(0, a, b);
+    for (int s = 0; s < steps; s++) {
+        bf8 an[TM], bn[TN];
+        if (s + 1 < steps) load(s + 1, an, bn);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
+#pragma unroll
+        for (int i = 0; i < TM; i++) a[i] = an[i];
+#pragma unroll
+        for (int j = 0; j < TN; j++) b[j] = bn[j];
+    }
+    bf16* T = tile[w];
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+        const float bv = bias[col0 + 16 * j + r16];
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float v = acc[i][j][e] + bv;
+                T[(16 * i + 4 * kg + e) * LDT + 16 * j + r16] = f2bf(v > 0.0f ? v : 0.0f);
+            }
+    }
+    __syncthreads();
+    constexpr int CPR = WN / 8;          // 16-byte chunks per row
+    constexpr int RPI = 64 / CPR;        // rows per wave instruction
+#pragma unroll
+    for (int r = 0; r < WM; r += RPI) {
+        const int rr = r + lane / CPR, c = lane % CPR;
+        const int m = row0 + rr;
+        const uint4 v = *(const uint4*)(T + rr * LDT + 8 * c);
+        if (m < M) *(uint4*)(Y + (size_t)m * g.Co + col0 + 8 * c) = v;
     }
 }
 
@@ -748,12 +867,40 @@ extern "C" int rt_cnn_get_params(rt_cnn* net, float* params) {
     return RT_OK;
 }
 
+// One forward layer on the 16x16x32 MFMA kernel when its shape fits: Cin = 4
+// or a power of two >= 8, Cout a multiple of 32 (Cin = 4) or 64. The wave tile
+// is the tallest of 64, 32, 16 rows that still gives 2048 waves. Returns false
+// when the layer needs the general kernels.
+static bool forward_mfma(const ConvGeom& g, const bf16* X, const bf16* Wp, const float* bias, bf16* Y, hipStream_t s) {
+    const int M = g.B * g.Ho * g.Wo;
+    const bool pow2 = g.Ci >= 8 && (g.Ci & (g.Ci - 1)) == 0;
+    if (g.Ci == 4 && g.Co % 32 == 0) {
+        k_conv_fwd_mfma<4, 2, 4, 1, true><<<dim3((M + 255) / 256, g.Co / 32), 256, 0, s>>>(g, X, Wp, bias, Y);
+        return true;
+    }
+    if (!pow2 || g.Co % 64) return false;
+    auto waves = [&](int wm) { return (long)((M + wm - 1) / wm) * (g.Co / 64); };
+    if (g.Co == 64) {  // one column tile: the four waves stacked over rows
+        if (waves(64) >= 2048) k_conv_fwd_mfma<4, 4, 4, 1, false><<<dim3((M + 255) / 256, 1), 256, 0, s>>>(g, X, Wp, bias, Y);
+        else if (waves(32) >= 2048) k_conv_fwd_mfma<2, 4, 4, 1, false><<<dim3((M + 127) / 128, 1), 256, 0, s>>>(g, X, Wp, bias, Y);
+        else k_conv_fwd_mfma<1, 4, 4, 1, false><<<dim3((M + 63) / 64, 1), 256, 0, s>>>(g, X, Wp, bias, Y);
+        return true;
+    }
+    if (g.Co % 128) return false;
+    const dim3 gy(0, g.Co / 128);
+    if (waves(64) >= 2048) k_conv_fwd_mfma<4, 4, 2, 2, false><<<dim3((M + 127) / 128, gy.y), 256, 0, s>>>(g, X, Wp, bias, Y);
+    else if (waves(32) >= 2048) k_conv_fwd_mfma<2, 4, 2, 2, false><<<dim3((M + 63) / 64, gy.y), 256, 0, s>>>(g, X, Wp, bias, Y);
+    else k_conv_fwd_mfma<1, 4, 2, 2, false><<<dim3((M + 31) / 32, gy.y), 256, 0, s>>>(g, X, Wp, bias, Y);
+    return true;
+}
+
 static int forward(rt_cnn* net, int n, const bf16* x, hipStream_t s) {
     if (!net->packed) return set_error(RT_ERR_STATE, "rt_cnn: parameters not set");
     net->act[0] = const_cast<bf16*>(x);
     for (int l = 0; l < 6; l++) {
         ConvGeom g = geom(net, l, n);
         int M = n * g.Ho * g.Wo;
+        if (forward_mfma(g, net->act[l], net->Wp[l], net->params + net->b_off[l], net->act[l + 1], s)) continue;
         if (g.Ci % FT_K == 0 && g.Co % FT_N == 0) {
             dim3 grid((M + FT_M - 1) / FT_M, g.Co / FT_N);
             k_conv_fwd_lds<<<grid, 256, 0, s>>>(g, net->act[l], net->Wp[l], net->params + net->b_off[l],
