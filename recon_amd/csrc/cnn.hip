@@ -322,6 +322,134 @@ __global__ __launch_bounds__(256) void k_conv_fwd_mfma(ConvGeom g, const bf16* _
     }
 }
 
+// Forward for Cin a multiple of 32 (the layers after the first): a workgroup
+// tile of BM output pixels x BN output channels, k in steps of 32 (one tap, 32
+// channels). Each step's A rows (64-byte runs of the input) and B rows (64
+// bytes of weights) are staged in LDS by the whole workgroup, double-buffered:
+// the next step's global loads are issued before this step's MFMAs and stored
+// to the other buffer after them, one barrier per step. Waves (WGM x 4/WGM)
+// take 16*TM x 16*TN sub-tiles with v_mfma_f32_16x16x32_bf16; the epilogue is
+// the one of k_conv_fwd_mfma.
+template <int BM, int BN, int WGM>
+__global__ __launch_bounds__(256) void k_conv_fwd_lds32(ConvGeom g, const bf16* __restrict__ X,
+                                                        const bf16* __restrict__ Wp, const float* __restrict__ bias,
+                                                        bf16* __restrict__ Y) {
+    constexpr int WGN = 4 / WGM, WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
+    constexpr int LD = 40;  // LDS row pitch in bf16: 32 k + 8 (rows start 80 bytes apart)
+    constexpr int STAGE = (BM + BN) * LD;
+    constexpr int AC = BM * 4, BC = BN * 4;  // 16-byte chunks per stage
+    constexpr int AQ = (AC + 255) / 256, BQ = (BC + 255) / 256;
+    constexpr int TLD = BN + 8;  // epilogue tile pitch
+    static_assert(BM * TLD <= 2 * STAGE, "epilogue tile fits in the stage buffers");
+    __shared__ __attribute__((aligned(16))) bf16 lds[2 * STAGE];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, r16 = lane & 15, kg = lane >> 4;
+    const int M = g.B * g.Ho * g.Wo, HW = g.Ho * g.Wo;
+    const int bm0 = blockIdx.x * BM, bn0 = blockIdx.y * BN;
+    int arow[AQ], apart[AQ], ayb[AQ], axb[AQ];
+    size_t anb[AQ];
+    bool av[AQ], ae[AQ];
+#pragma unroll
+    for (int q = 0; q < AQ; q++) {
+        const int e = t + 256 * q;
+        ae[q] = e < AC;
+        arow[q] = (e >> 2) % BM;
+        apart[q] = e & 3;
+        const int m = bm0 + arow[q];
+        av[q] = ae[q] && m < M;
+        const int mm = av[q] ? m : 0;
+        const int n = mm / HW, r = mm - n * HW, yo = r / g.Wo, xo = r - yo * g.Wo;
+        ayb[q] = 2 * yo - 1;
+        axb[q] = 2 * xo - 1;
+        anb[q] = (size_t)n * g.Hi * g.Wi;
+    }
+    int brow[BQ], bpart[BQ];
+    bool be[BQ];
+    const bf16* bp[BQ];
+#pragma unroll
+    for (int q = 0; q < BQ; q++) {
+        const int e = t + 256 * q;
+        be[q] = e < BC;
+        brow[q] = (e >> 2) % BN;
+        bpart[q] = e & 3;
+        bp[q] = Wp + (size_t)(bn0 + brow[q]) * g.Kp + 8 * bpart[q];
+    }
+    uint4 ra[AQ], rb[BQ];
+    auto gload = [&](int s) {
+        const int k = 32 * s;
+        const int tap = k >> g.ci_shift, ci0 = k & (g.Ci - 1);
+        const int ky = tap / 3, kx = tap - 3 * (tap / 3);
+#pragma unroll
+        for (int q = 0; q < AQ; q++) {
+            const int yi = ayb[q] + ky, xi = axb[q] + kx;
+            const bool ok = av[q] && yi >= 0 && yi < g.Hi && xi >= 0 && xi < g.Wi;
+            ra[q] = ok ? *(const uint4*)(X + (anb[q] + (size_t)yi * g.Wi + xi) * g.Ci + ci0 + 8 * apart[q])
+                       : make_uint4(0u, 0u, 0u, 0u);
+        }
+#pragma unroll
+        for (int q = 0; q < BQ; q++) rb[q] = be[q] ? *(const uint4*)(bp[q] + k) : make_uint4(0u, 0u, 0u, 0u);
+    };
+    auto sstore = [&](int buf) {
+        bf16* A = lds + buf * STAGE;
+        bf16* Bs = A + BM * LD;
+#pragma unroll
+        for (int q = 0; q < AQ; q++)
+            if (ae[q]) *(uint4*)(A + arow[q] * LD + 8 * apart[q]) = ra[q];
+#pragma unroll
+        for (int q = 0; q < BQ; q++)
+            if (be[q]) *(uint4*)(Bs + brow[q] * LD + 8 * bpart[q]) = rb[q];
+    };
+    const int wm0 = (w % WGM) * WM, wn0 = (w / WGM) * WN;
+    f4 acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
+    const int steps = g.Kp / 32;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int s = 0; s < steps; s++) {
+        const int cur = s & 1;
+        if (s + 1 < steps) gload(s + 1);
+        const bf16* A = lds + cur * STAGE;
+        const bf16* Bs = A + BM * LD;
+        bf8 a[TM], b[TN];
+#pragma unroll
+        for (int i = 0; i < TM; i++) a[i] = *(const bf8*)(A + (wm0 + 16 * i + r16) * LD + 8 * kg);
+#pragma unroll
+        for (int j = 0; j < TN; j++) b[j] = *(const bf8*)(Bs + (wn0 + 16 * j + r16) * LD + 8 * kg);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
+        if (s + 1 < steps) sstore(cur ^ 1);
+        __syncthreads();
+    }
+    bf16* T = lds;
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+        const float bv = bias[bn0 + wn0 + 16 * j + r16];
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const float v = acc[i][j][e] + bv;
+                T[(wm0 + 16 * i + 4 * kg + e) * TLD + wn0 + 16 * j + r16] = f2bf(v > 0.0f ? v : 0.0f);
+            }
+    }
+    __syncthreads();
+    constexpr int CPR = BN / 8;
+#pragma unroll
+    for (int e0 = 0; e0 < BM * CPR; e0 += 256) {
+        const int e = e0 + t;
+        if (e < BM * CPR) {
+            const int rr = e / CPR, c = e - rr * CPR;
+            const int m = bm0 + rr;
+            if (m < M) *(uint4*)(Y + (size_t)m * g.Co + bn0 + 8 * c) = *(const uint4*)(T + rr * TLD + 8 * c);
+        }
+    }
+}
+
 // ---------------------------------------------------------------- input gradient
 // dX = conv^T(dZ, W), masked by the ReLU of the layer input (X > 0), stored as
 // the next (earlier) layer's dZ. WT: [Cin][9][Cout].
@@ -879,6 +1007,22 @@ static bool forward_mfma(const ConvGeom& g, const bf16* X, const bf16* Wp, const
         return true;
     }
     if (!pow2 || g.Co % 64) return false;
+    if (g.Ci % 32 == 0) {
+        // LDS-staged tiles: the largest workgroup tile that still gives 512 workgroups
+        auto blocks = [&](int bm, int bn) { return (long)((M + bm - 1) / bm) * (g.Co / bn); };
+        if (g.Co == 64) {
+            if (blocks(256, 64) >= 512) k_conv_fwd_lds32<256, 64, 4><<<dim3((M + 255) / 256, 1), 256, 0, s>>>(g, X, Wp, bias, Y);
+            else k_conv_fwd_lds32<128, 64, 4><<<dim3((M + 127) / 128, 1), 256, 0, s>>>(g, X, Wp, bias, Y);
+            return true;
+        }
+        if (g.Co % 128 == 0) {
+            const unsigned gy = g.Co / 128;
+            if (blocks(128, 128) >= 512) k_conv_fwd_lds32<128, 128, 2><<<dim3((M + 127) / 128, gy), 256, 0, s>>>(g, X, Wp, bias, Y);
+            else if (blocks(64, 128) >= 512) k_conv_fwd_lds32<64, 128, 2><<<dim3((M + 63) / 64, gy), 256, 0, s>>>(g, X, Wp, bias, Y);
+            else k_conv_fwd_lds32<32, 128, 2><<<dim3((M + 31) / 32, gy), 256, 0, s>>>(g, X, Wp, bias, Y);
+            return true;
+        }
+    }
     auto waves = [&](int wm) { return (long)((M + wm - 1) / wm) * (g.Co / 64); };
     if (g.Co == 64) {  // one column tile: the four waves stacked over rows
         if (waves(64) >= 2048) k_conv_fwd_mfma<4, 4, 4, 1, false><<<dim3((M + 255) / 256, 1), 256, 0, s>>>(g, X, Wp, bias, Y);

@@ -55,11 +55,11 @@ def case(cfg, B, seed):
     return p, xr, xb, y, wn
 
 
-@pytest.mark.parametrize("kw,B", [({}, 3), (SMALL, 5)])
+@pytest.mark.parametrize("kw,B", [({}, 3), ({}, 37), (SMALL, 5)])
 def test_forward_matches_oracle(kw, B):
     print(f"case cnn forward {kw} B={B}", flush=True)
     cfg = default_config(**kw)
-    net = PoseNet(8, config=cfg)
+    net = PoseNet(max(8, B), config=cfg)
     p, xr, xb, y, wn = case(cfg, B, 1)
     net.set_params(p)
     xd = DeviceBuffer.from_array(xb)
