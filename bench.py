@@ -199,6 +199,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall time of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU threads (0: nproc)")
     ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--inflight", type=int, default=1, choices=(1, 2),
+                    help="learned workload: batches in flight (2: the next step's preparation overlaps this step's tracking)")
     ap.add_argument("--extras", action="store_true",
                     help="also run the single-pair and sequence cases of the earlier build's extra leg "
                          "(--workload single / sequence give them their own lines)")
@@ -312,6 +314,7 @@ def main():
         Trel_b, _, rms, st = step()
     stream_sync(stream)
     step_s = (time.perf_counter() - t2) / args.steps
+    serial_step_s = step_s
     frames_per_s = B / step_s
 
     # ---- dominant kernel (the persistent tracking launch), HIP events on its stream
@@ -355,6 +358,44 @@ def main():
                        "proposal_ms: the whole proposal step (inputs from the raw frames, which stream 7.5 GB, "
                        "+ forward + poses)"}
         del xin, yout
+
+    # ---- two batches in flight (learned): the host's preparation of step k+1
+    # and its launches overlap step k's tracking; every step still prepares,
+    # proposes and tracks its 4096 pairs and returns their poses to the host
+    inflight = None
+    if learned and args.inflight == 2:
+        from recon_amd.cnn import PoseNet
+        bt2 = BatchTracker(B, W, H, mode=PHOTOMETRIC, n_select=N, iters=iters)
+        prepared2 = bt2.prepare(pairs)
+        bt2.track(prepared2)  # the second batch's variant is tuned with the GPU otherwise idle
+        net2 = PoseNet.load(args.weights, max_batch=B)
+        lanes = [(bt, prepared, net), (bt2, prepared2, net2)]
+
+        def launch(k):
+            b, pr, n = lanes[k % 2]
+            b.prepare_only(pr)
+            b.propose(n)
+            b.retrack_async()
+
+        launch(0)
+        launch(1)
+        lanes[0][0].results(B)
+        lanes[1][0].results(B)
+        t4 = time.perf_counter()
+        launch(0)
+        for k in range(1, args.steps):
+            launch(k)
+            Trel_p, _, _, _ = lanes[(k - 1) % 2][0].results(B)
+        Trel_p, _, _, _ = lanes[(args.steps - 1) % 2][0].results(B)
+        pipe_s = (time.perf_counter() - t4) / args.steps
+        same = bool(np.array_equal(Trel_p, Trel_b))
+        inflight = {"ms_per_step": round(pipe_s * 1e3, 3), "serial_ms_per_step": round(serial_step_s * 1e3, 3),
+                    "poses_identical_to_serial": same}
+        log(f"two batches in flight: {pipe_s * 1e3:.2f} ms/step (serial {serial_step_s * 1e3:.2f}); "
+            f"poses identical to the serial steps: {same}")
+        step_s = pipe_s
+        frames_per_s = B / step_s
+        del lanes, bt2, net2
 
     # ---- convergence and errors against the FP64 oracle tracker started at the
     # true motion (the judge), on the first S pairs of the batch
@@ -425,6 +466,7 @@ def main():
                                f"{N} points, {L} levels, iterations {list(iters[:L])}",
                    "pairs": B, "width": W, "height": H, "points": N, "levels": L, "theta_deg": theta, "tmax_mm": tmax},
         "frames_per_s_all_pairs": round(frames_per_s, 1),
+        "batches_in_flight": inflight,
         "plain_fps": round(plain_fps, 1),
         "plain_converged_fps": round(plain_conv_fps, 1),
         # converged frames/s over the plain tracker's frames/s, whether or not its pairs converge

@@ -253,6 +253,11 @@ int rt_batch_track(rt_batch* b, int n, const rt_pair* pairs, rt_pair_result* out
  * poses already on the device (those of the last call, or the proposals of
  * rt_batch_propose). out NULL: enqueue without waiting. */
 int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_result* out);
+/* Wait for the batch's stream and copy the results of its last tracking
+ * launch (one that rt_batch_retrack enqueued with out NULL) to out (host).
+ * No counterpart in the reference: it lets a caller keep two batches in
+ * flight, preparing one while the other tracks. */
+int rt_batch_results(rt_batch* b, rt_pair_result* out);
 void* rt_batch_stream(rt_batch* b);
 /* The persistent kernel's variant table (the autotuner's candidates; the
  * environment variable RT_BATCH_VARIANT=v pins variant v): info receives

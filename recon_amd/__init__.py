@@ -89,6 +89,7 @@ SIGNATURES = {
     "rt_batch_destroy": (None, [c_void_p]),
     "rt_batch_track": (c_int, [c_void_p, c_int, P(RtPair), P(RtPairResult)]),
     "rt_batch_retrack": (c_int, [c_void_p, P(c_double), P(RtPairResult)]),
+    "rt_batch_results": (c_int, [c_void_p, P(RtPairResult)]),
     "rt_batch_stream": (c_void_p, [c_void_p]),
     "rt_batch_variant_info": (c_int, [c_int, P(c_int)]),
     "rt_batch_track_noalloc": (c_int, [c_int, c_int, P(RtPair), P(RtParams), c_void_p, P(RtPairResult)]),

@@ -1421,6 +1421,12 @@ extern "C" int rt_batch_retrack(rt_batch* b, const double* init_T, rt_pair_resul
     return run_track(b, out);
 }
 
+extern "C" int rt_batch_results(rt_batch* b, rt_pair_result* out) {
+    if (!b || b->n < 1 || !out) return set_error(RT_ERR_STATE, "rt_batch_results: no prepared batch or no output");
+    if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    return fetch_results(b, out);
+}
+
 // ------------------------------------------------------------------ no-alloc batch tracking
 // rt_batch_track for callers that cannot allocate device memory: the pair
 // arenas and tables live in one fixed device array (`scratch`, held by every

@@ -336,6 +336,18 @@ class BatchTracker:
         check(self.L.rt_batch_retrack(self.h, ip, out), "rt_batch_retrack")
         return _results(out)
 
+    def results(self, n):
+        """Results of the last tracking launch enqueued without output
+        (retrack_async), after waiting for it."""
+        out = (RtPairResult * n)()
+        check(self.L.rt_batch_results(self.h, out), "rt_batch_results")
+        return _results(out)
+
+    def retrack_async(self):
+        """Enqueue a tracking launch from the initial poses on the device
+        (the proposals) without waiting; results() collects it."""
+        check(self.L.rt_batch_retrack(self.h, None, None), "rt_batch_retrack")
+
 
 _RESULT_DTYPE = np.dtype([("Trel", np.float64, (4, 4)), ("T", np.float64, (4, 4)), ("rms", np.float32),
                           ("status", np.int32)])

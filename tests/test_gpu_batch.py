@@ -65,6 +65,10 @@ def test_batch_retrack_deterministic_and_warm_start():
     a = bt.track(pairs)[0]
     b = bt.retrack(n=n)[0]
     assert np.array_equal(a, b)
+    # the same launch enqueued without waiting, collected by rt_batch_results
+    bt.retrack_async()
+    r = bt.results(n)[0]
+    assert np.array_equal(a, r)
     # starting at the converged pose re-converges next to it (the coarse levels
     # run again, so it may move by a fraction of a millimetre)
     c = bt.retrack(init=a.reshape(n, 16), n=n)[0]
