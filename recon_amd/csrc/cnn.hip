@@ -450,6 +450,139 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds32(ConvGeom g, const bf16* 
     }
 }
 
+// The first two forward layers fused, one workgroup (8 waves) per pair: the
+// first layer (4 -> C1 channels, direct 8-byte tap loads as in
+// k_conv_fwd_mfma) writes its H1 x W1 x C1 output to LDS with bias, ReLU and
+// bfloat16 rounding; after a barrier the second layer (C1 = 32 -> C2) reads
+// its A operands from that LDS map (one 16-byte run per lane and tap) and its
+// weights from global memory. The intermediate never reaches HBM; the
+// second layer's tile leaves through LDS as 16-byte row chunks. Two
+// workgroups fit in a CU's LDS.
+constexpr int F12_C1 = 32;
+constexpr int F12_LDS = 30 * 40 * F12_C1;  // bf16 elements: the default network's first-layer map
+template <int C2>
+__global__ __launch_bounds__(512) void k_conv12_fused(ConvGeom g1, ConvGeom g2, const bf16* __restrict__ X,
+                                                      const bf16* __restrict__ W1, const float* __restrict__ b1,
+                                                      const bf16* __restrict__ W2, const float* __restrict__ b2,
+                                                      bf16* __restrict__ Y) {
+    constexpr int NW = 8, TN2 = C2 / 16, C2LD = C2 + 8;
+    __shared__ __attribute__((aligned(16))) bf16 map[F12_LDS];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6, r16 = lane & 15, kg = lane >> 4;
+    const int pair = blockIdx.x;
+    const int H1 = g1.Ho, Wd1 = g1.Wo, M1 = H1 * Wd1, H2 = g2.Ho, Wd2 = g2.Wo, M2 = H2 * Wd2;
+    const bf16* Xp = X + (size_t)pair * g1.Hi * g1.Wi * 4;
+    // ---- first layer: K = 36 in two steps of 32 (taps 8s + 2kg, 8s + 2kg + 1)
+    bf8 bw[2][2];
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+            bw[j][s] = __builtin_bit_cast(bf8, *(const uint4*)(W1 + (size_t)(16 * j + r16) * g1.Kp + 32 * s + 8 * kg));
+    float bias1[2];
+#pragma unroll
+    for (int j = 0; j < 2; j++) bias1[j] = b1[16 * j + r16];
+    const int RT1 = (M1 + 15) / 16;
+    constexpr int G1 = 4;  // row tiles whose loads are in flight together
+    for (int rt0 = w; rt0 < RT1; rt0 += G1 * NW) {
+        bf8 a[G1][2];
+#pragma unroll
+        for (int u = 0; u < G1; u++) {
+            const int m = (rt0 + NW * u) * 16 + r16;
+            const bool mv = rt0 + NW * u < RT1 && m < M1;
+            const int yb = 2 * (m / Wd1) - 1, xb = 2 * (m % Wd1) - 1;
+#pragma unroll
+            for (int s = 0; s < 2; s++) {
+                uint2 v[2];
+#pragma unroll
+                for (int h = 0; h < 2; h++) {
+                    const int tap = 8 * s + 2 * kg + h;
+                    const int yi = yb + tap / 3, xi = xb + tap % 3;
+                    const bool ok = mv && tap < 9 && yi >= 0 && yi < g1.Hi && xi >= 0 && xi < g1.Wi;
+                    v[h] = ok ? *(const uint2*)(Xp + ((size_t)yi * g1.Wi + xi) * 4) : make_uint2(0u, 0u);
+                }
+                a[u][s] = __builtin_bit_cast(bf8, make_uint4(v[0].x, v[0].y, v[1].x, v[1].y));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < G1; u++) {
+            const int rt = rt0 + NW * u;
+            if (rt >= RT1) continue;
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+                acc = mfma32(a[u][0], bw[j][0], acc);
+                acc = mfma32(a[u][1], bw[j][1], acc);
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int mr = rt * 16 + 4 * kg + e;
+                    const float v = acc[e] + bias1[j];
+                    if (mr < M1) map[mr * F12_C1 + 16 * j + r16] = f2bf(v > 0.0f ? v : 0.0f);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // ---- second layer: K = 9 taps x 32 channels, one tap per step
+    const int RT2 = (M2 + 15) / 16;
+    constexpr int MAXRT = 3;  // row tiles per wave: the default network has 19 over 8 waves
+    f4 acc[MAXRT][TN2];
+#pragma unroll
+    for (int q = 0; q < MAXRT; q++)
+#pragma unroll
+        for (int j = 0; j < TN2; j++) acc[q][j] = f4{0.f, 0.f, 0.f, 0.f};
+    int yb[MAXRT], xb[MAXRT];
+    bool mv[MAXRT];
+#pragma unroll
+    for (int q = 0; q < MAXRT; q++) {
+        const int m = (w + NW * q) * 16 + r16;
+        mv[q] = (w + NW * q) < RT2 && m < M2;
+        yb[q] = 2 * (m / Wd2) - 1;
+        xb[q] = 2 * (m % Wd2) - 1;
+    }
+    const bool any1 = w + NW < RT2, any2 = w + 2 * NW < RT2;
+    for (int s = 0; s < 9; s++) {
+        const int ky = s / 3, kx = s % 3;
+        bf8 b[TN2];
+#pragma unroll
+        for (int j = 0; j < TN2; j++)
+            b[j] = __builtin_bit_cast(bf8, *(const uint4*)(W2 + (size_t)(16 * j + r16) * g2.Kp + 32 * s + 8 * kg));
+#pragma unroll
+        for (int q = 0; q < MAXRT; q++) {
+            if ((q == 1 && !any1) || (q == 2 && !any2)) continue;
+            const int yi = yb[q] + ky, xi = xb[q] + kx;
+            const bool ok = mv[q] && yi >= 0 && yi < H1 && xi >= 0 && xi < Wd1;
+            const uint4 v = ok ? *(const uint4*)(map + (yi * Wd1 + xi) * F12_C1 + 8 * kg) : make_uint4(0u, 0u, 0u, 0u);
+            const bf8 a = __builtin_bit_cast(bf8, v);
+#pragma unroll
+            for (int j = 0; j < TN2; j++) acc[q][j] = mfma32(a, b[j], acc[q][j]);
+        }
+    }
+    __syncthreads();  // the map is read by every wave; its LDS now holds the output tile
+    bf16* T = map;
+#pragma unroll
+    for (int q = 0; q < MAXRT; q++) {
+        const int rt = w + NW * q;
+        if (rt >= RT2) continue;
+#pragma unroll
+        for (int j = 0; j < TN2; j++) {
+            const float bv = b2[16 * j + r16];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                const int mr = rt * 16 + 4 * kg + e;
+                const float v = acc[q][j][e] + bv;
+                if (mr < M2) T[mr * C2LD + 16 * j + r16] = f2bf(v > 0.0f ? v : 0.0f);
+            }
+        }
+    }
+    __syncthreads();
+    constexpr int CPR = C2 / 8;
+    bf16* Yp = Y + (size_t)pair * M2 * C2;
+    for (int e = t; e < M2 * CPR; e += 512) {
+        const int rr = e / CPR, c = e - rr * CPR;
+        *(uint4*)(Yp + (size_t)rr * C2 + 8 * c) = *(const uint4*)(T + rr * C2LD + 8 * c);
+    }
+}
+
 // ---------------------------------------------------------------- input gradient
 // dX = conv^T(dZ, W), masked by the ReLU of the layer input (X > 0), stored as
 // the next (earlier) layer's dZ. WT: [Cin][9][Cout].
@@ -1038,10 +1171,28 @@ static bool forward_mfma(const ConvGeom& g, const bf16* X, const bf16* Wp, const
     return true;
 }
 
-static int forward(rt_cnn* net, int n, const bf16* x, hipStream_t s) {
+// The first two layers in one launch when their shapes are the fused
+// kernel's: 4 -> 32 -> 64 channels, the first layer's map within its LDS, the
+// second layer's rows within 3 row tiles per wave.
+static bool forward_fused12(rt_cnn* net, int n, hipStream_t s) {
+    const ConvGeom g1 = geom(net, 0, n), g2 = geom(net, 1, n);
+    if (g1.Ci != 4 || g1.Co != F12_C1 || g1.Kp != 64 || g2.Ci != F12_C1 || g2.Co != 64 || g2.Kp != 9 * F12_C1 ||
+        g2.Hi != g1.Ho || g2.Wi != g1.Wo || g1.Ho * g1.Wo * F12_C1 > F12_LDS ||
+        g2.Ho * g2.Wo * (64 + 8) > F12_LDS || (g2.Ho * g2.Wo + 15) / 16 > 3 * 8)
+        return false;
+    k_conv12_fused<64><<<n, 512, 0, s>>>(g1, g2, net->act[0], net->Wp[0], net->params + net->b_off[0], net->Wp[1],
+                                         net->params + net->b_off[1], net->act[2]);
+    return true;
+}
+
+// keep: every layer's activation stays in net->act (training's backward pass
+// reads them); otherwise the first two layers may run fused and the first
+// layer's activation is not stored.
+static int forward(rt_cnn* net, int n, const bf16* x, hipStream_t s, bool keep = false) {
     if (!net->packed) return set_error(RT_ERR_STATE, "rt_cnn: parameters not set");
     net->act[0] = const_cast<bf16*>(x);
-    for (int l = 0; l < 6; l++) {
+    const bool fused = !keep && forward_fused12(net, n, s);
+    for (int l = fused ? 2 : 0; l < 6; l++) {
         ConvGeom g = geom(net, l, n);
         int M = n * g.Ho * g.Wo;
         if (forward_mfma(g, net->act[l], net->Wp[l], net->params + net->b_off[l], net->act[l + 1], s)) continue;
@@ -1059,7 +1210,7 @@ static int forward(rt_cnn* net, int n, const bf16* x, hipStream_t s) {
 
 // forward + backward into net->grads; loss into net->loss[0] (device)
 static int loss_grad(rt_cnn* net, int n, const bf16* x, const float* y, const float* wn, hipStream_t s) {
-    int rc = forward(net, n, x, s);
+    int rc = forward(net, n, x, s, true);
     if (rc) return rc;
     k_weight_norm<<<1, 64, 0, s>>>(wn, n, net->loss + 1);
     const int C6 = net->cfg.channels[6], P6 = net->H[6] * net->W[6];
