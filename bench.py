@@ -279,7 +279,7 @@ def main():
         log(f"track-only: {args.steps} tracking launches over {B} pairs done")
         return
 
-    # ---- plain single-pair tracker (same kernels, one pair at a time, identity start)
+    # ---- plain single-pair tracker (its own kernels, rt_kernels.hip; one pair at a time, identity start)
     P = min(args.plain_pairs, B)
     tr = PhotoTracker(W, H, cal, mode=PHOTOMETRIC, n_select=N, iters=iters)
     plain_T = []
