@@ -330,14 +330,15 @@ __global__ __launch_bounds__(256) void k_conv_fwd_mfma(ConvGeom g, const bf16* _
 // to the other buffer after them, one barrier per step. Waves (WGM x 4/WGM)
 // take 16*TM x 16*TN sub-tiles with v_mfma_f32_16x16x32_bf16; the epilogue is
 // the one of k_conv_fwd_mfma.
-template <int BM, int BN, int WGM>
+template <int BM, int BN, int WGM, int KS = 32>
 __global__ __launch_bounds__(256) void k_conv_fwd_lds32(ConvGeom g, const bf16* __restrict__ X,
                                                         const bf16* __restrict__ Wp, const float* __restrict__ bias,
                                                         bf16* __restrict__ Y) {
     constexpr int WGN = 4 / WGM, WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
-    constexpr int LD = 40;  // LDS row pitch in bf16: 32 k + 8 (rows start 80 bytes apart)
+    constexpr int LD = KS + 8;  // LDS row pitch in bf16: KS k + 8
     constexpr int STAGE = (BM + BN) * LD;
-    constexpr int AC = BM * 4, BC = BN * 4;  // 16-byte chunks per stage
+    constexpr int CR = KS / 8;  // 16-byte chunks per row and stage
+    constexpr int AC = BM * CR, BC = BN * CR;
     constexpr int AQ = (AC + 255) / 256, BQ = (BC + 255) / 256;
     constexpr int TLD = BN + 8;  // epilogue tile pitch
     static_assert(BM * TLD <= 2 * STAGE, "epilogue tile fits in the stage buffers");
@@ -352,8 +353,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds32(ConvGeom g, const bf16* 
     for (int q = 0; q < AQ; q++) {
         const int e = t + 256 * q;
         ae[q] = e < AC;
-        arow[q] = (e >> 2) % BM;
-        apart[q] = e & 3;
+        arow[q] = (e / CR) % BM;
+        apart[q] = e % CR;
         const int m = bm0 + arow[q];
         av[q] = ae[q] && m < M;
         const int mm = av[q] ? m : 0;
@@ -369,13 +370,13 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds32(ConvGeom g, const bf16* 
     for (int q = 0; q < BQ; q++) {
         const int e = t + 256 * q;
         be[q] = e < BC;
-        brow[q] = (e >> 2) % BN;
-        bpart[q] = e & 3;
+        brow[q] = (e / CR) % BN;
+        bpart[q] = e % CR;
         bp[q] = Wp + (size_t)(bn0 + brow[q]) * g.Kp + 8 * bpart[q];
     }
     uint4 ra[AQ], rb[BQ];
     auto gload = [&](int s) {
-        const int k = 32 * s;
+        const int k = KS * s;
         const int tap = k >> g.ci_shift, ci0 = k & (g.Ci - 1);
         const int ky = tap / 3, kx = tap - 3 * (tap / 3);
 #pragma unroll
@@ -404,7 +405,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds32(ConvGeom g, const bf16* 
     for (int i = 0; i < TM; i++)
 #pragma unroll
         for (int j = 0; j < TN; j++) acc[i][j] = f4{0.f, 0.f, 0.f, 0.f};
-    const int steps = g.Kp / 32;
+    const int steps = g.Kp / KS;
     gload(0);
     sstore(0);
     __syncthreads();
@@ -413,15 +414,18 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds32(ConvGeom g, const bf16* 
         if (s + 1 < steps) gload(s + 1);
         const bf16* A = lds + cur * STAGE;
         const bf16* Bs = A + BM * LD;
-        bf8 a[TM], b[TN];
 #pragma unroll
-        for (int i = 0; i < TM; i++) a[i] = *(const bf8*)(A + (wm0 + 16 * i + r16) * LD + 8 * kg);
+        for (int kk = 0; kk < KS; kk += 32) {
+            bf8 a[TM], b[TN];
 #pragma unroll
-        for (int j = 0; j < TN; j++) b[j] = *(const bf8*)(Bs + (wn0 + 16 * j + r16) * LD + 8 * kg);
+            for (int i = 0; i < TM; i++) a[i] = *(const bf8*)(A + (wm0 + 16 * i + r16) * LD + kk + 8 * kg);
 #pragma unroll
-        for (int i = 0; i < TM; i++)
+            for (int j = 0; j < TN; j++) b[j] = *(const bf8*)(Bs + (wn0 + 16 * j + r16) * LD + kk + 8 * kg);
 #pragma unroll
-            for (int j = 0; j < TN; j++) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) acc[i][j] = mfma32(a[i], b[j], acc[i][j]);
+        }
         if (s + 1 < steps) sstore(cur ^ 1);
         __syncthreads();
     }
@@ -1152,6 +1156,8 @@ static bool forward_mfma(const ConvGeom& g, const bf16* X, const bf16* Wp, const
             const unsigned gy = g.Co / 128;
             if (blocks(128, 128) >= 512) k_conv_fwd_lds32<128, 128, 2><<<dim3((M + 127) / 128, gy), 256, 0, s>>>(g, X, Wp, bias, Y);
             else if (blocks(64, 128) >= 512) k_conv_fwd_lds32<64, 128, 2><<<dim3((M + 63) / 64, gy), 256, 0, s>>>(g, X, Wp, bias, Y);
+            else if (g.Ci % 64 == 0)  // few rows and a long k: stages of 64 halve its barriers and load round trips
+                k_conv_fwd_lds32<32, 128, 2, 64><<<dim3((M + 31) / 32, gy), 256, 0, s>>>(g, X, Wp, bias, Y);
             else k_conv_fwd_lds32<32, 128, 2><<<dim3((M + 31) / 32, gy), 256, 0, s>>>(g, X, Wp, bias, Y);
             return true;
         }
