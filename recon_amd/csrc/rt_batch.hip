@@ -80,13 +80,14 @@ __device__ __forceinline__ void load_lut(float* lut) {
     for (int t = threadIdx.x; t < 256; t += blockDim.x) lut[t] = c_u8_table.v[t];
 }
 
-// pyramid levels 1..L-1 of one frame (fastDownSampleHdrKernel, downSample2Cuda.cu:35-42)
+// pyramid levels 1..L-1 of one frame (fastDownSampleHdrKernel, downSample2Cuda.cu:35-42);
+// the keyframe's (which 0) are built by kb_bins from its gray tile
 __global__ __launch_bounds__(256) void kb_pyramid(const PairDesc* __restrict__ P, int which) {
     const PairDesc& d = P[blockIdx.z];
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 64 >= d.H) return;
     convert_pyramid_block<3>(blockIdx.x, blockIdx.y, d.g8[which], nullptr, nullptr, d.W, d.H, d.L, d.calib, nullptr,
                              d.gray[which][1], d.gray[which][2], d.gray[which][3], nullptr);
-    if (which == 0 && blockIdx.x == 0 && blockIdx.y == 0) d.ghist[threadIdx.x] = 0u;
+    if (which == 1 && blockIdx.x == 0 && blockIdx.y == 0) d.ghist[threadIdx.x] = 0u;  // for kb_bins
 }
 
 // xyz2CloudKernel's gradient magnitude (d2z.cu:235-331) as an 8-bit bin per
@@ -155,6 +156,41 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
         }
     }
     __syncthreads();
+    // the keyframe's pyramid levels 1..L-1 from the tile's core (64 x 16 level-0
+    // pixels: 32 x 8, 16 x 4, 8 x 2), in convert_pyramid_block's order
+    // ((top left + top right) + bottom left) + bottom right, then / 4
+    if (d.L > 1) {
+        __shared__ float l1s[8][32];
+        __shared__ float l2s[4][16];
+        const int W1 = W >> 1, H1 = H >> 1, W2 = W >> 2, H2 = H >> 2;
+        {
+            const int i = threadIdx.x & 31, j = threadIdx.x >> 5;
+            const float* q = gt + (2 + 2 * j) * TW + 4 + 2 * i;
+            const float v = (q[0] + q[1] + q[TW] + q[TW + 1]) / 4.0f;
+            l1s[j][i] = v;
+            const int X1 = blockIdx.x * 32 + i, Y1 = blockIdx.y * (BINS_TILE_H / 2) + j;
+            if (X1 < W1 && Y1 < H1) d.gray[0][1][(size_t)Y1 * W1 + X1] = v;
+        }
+        if (d.L > 2) {
+            __syncthreads();
+            if (threadIdx.x < 64) {
+                const int i = threadIdx.x & 15, j = threadIdx.x >> 4;
+                const float v = (l1s[2 * j][2 * i] + l1s[2 * j][2 * i + 1] + l1s[2 * j + 1][2 * i] + l1s[2 * j + 1][2 * i + 1]) / 4.0f;
+                l2s[j][i] = v;
+                const int X2 = blockIdx.x * 16 + i, Y2 = blockIdx.y * (BINS_TILE_H / 4) + j;
+                if (X2 < W2 && Y2 < H2) d.gray[0][2][(size_t)Y2 * W2 + X2] = v;
+            }
+            if (d.L > 3) {
+                __syncthreads();
+                if (threadIdx.x < 16) {
+                    const int i = threadIdx.x & 7, j = threadIdx.x >> 3;
+                    const float v = (l2s[2 * j][2 * i] + l2s[2 * j][2 * i + 1] + l2s[2 * j + 1][2 * i] + l2s[2 * j + 1][2 * i + 1]) / 4.0f;
+                    const int X3 = blockIdx.x * 8 + i, Y3 = blockIdx.y * (BINS_TILE_H / 8) + j;
+                    if (X3 < (W >> 3) && Y3 < (H >> 3)) d.gray[0][3][(size_t)Y3 * (W >> 3) + X3] = v;
+                }
+            }
+        }
+    }
     const LdsTile tile{gt, tx0, ty0, TW, TH};
     float cl[96];
     uniform_calib(cal, cl);
@@ -1042,13 +1078,13 @@ static BatchView view_of(const rt_batch* b) {
                      b->prm.solver, b->prm.robust, b->prm.mad_bins, b->prm.mode, b->prm.lambda};
 }
 
-// Keyframe preparation in two parts: the pyramids of both frames (part 1) and
-// the gradient bins, selection, points and clouds (part 2); 0: both.
+// Keyframe preparation in two parts: the current frame's pyramid (part 1) and
+// the gradient bins with the keyframe's pyramid, selection, points and clouds
+// (part 2); 0: both.
 static void launch_prep(const BatchView& v, hipStream_t s, int part = 0) {
     const int n = v.n;
     if (part != 2) {
         dim3 g64((v.maxW + 63) / 64, (v.maxH + 63) / 64, n);
-        kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 0);
         kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 1);
     }
     if (part == 1) return;
