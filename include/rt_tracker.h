@@ -346,6 +346,11 @@ int rt_debug_prepare(rt_tracker* t);
  * level l and/or the geometric one (iteration it: 0 re-selects and re-weights).
  * Results are readable through RT_DBG_SYSTEM, RT_DBG_RESIDUAL, RT_DBG_ICP_*. */
 int rt_debug_linearize(rt_tracker* t, int level, int it);
+/* The reciprocal the batched preparation takes of each 16-bit depth value:
+ * host[i] = RN(1 / -(i + 1)) as the device computes it, for i < n <= 65535.
+ * No counterpart in the reference (whose kernels divide); it lets a test
+ * compare the device reciprocal with IEEE division over every input. */
+int rt_debug_depth_reciprocals(int device, float* host, int n);
 
 const char* rt_last_error(void);
 

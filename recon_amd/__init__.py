@@ -99,6 +99,7 @@ SIGNATURES = {
     "rt_debug_set_trel": (c_int, [c_void_p, P(c_float)]),
     "rt_debug_prepare": (c_int, [c_void_p]),
     "rt_debug_linearize": (c_int, [c_void_p, c_int, c_int]),
+    "rt_debug_depth_reciprocals": (c_int, [c_int, P(c_float), c_int]),
     "rt_last_error": (ctypes.c_char_p, []),
     "rt_cnn_default_config": (None, [P(RtCnnConfig)]),
     "rt_cnn_create": (c_int, [c_int, P(RtCnnConfig), c_int, P(c_void_p)]),

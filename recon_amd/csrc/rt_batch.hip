@@ -37,6 +37,7 @@ struct PairDesc {
     int iters[4];
     const float* calib;
     int simple;             // calib_simple(calib), evaluated at preparation
+    float ifx, ify;         // RN(1 / fx), RN(1 / fy) of the depth camera (IEEE divisions on the host)
     const uint8_t* g8[2];
     const uint16_t* d16[2];
     float* gray[2][4];
@@ -87,7 +88,10 @@ __global__ __launch_bounds__(256) void kb_pyramid(const PairDesc* __restrict__ P
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 64 >= d.H) return;
     convert_pyramid_block<3>(blockIdx.x, blockIdx.y, d.g8[which], nullptr, nullptr, d.W, d.H, d.L, d.calib, nullptr,
                              d.gray[which][1], d.gray[which][2], d.gray[which][3], nullptr);
-    if (which == 1 && blockIdx.x == 0 && blockIdx.y == 0) d.ghist[threadIdx.x] = 0u;  // for kb_bins
+}
+// clears each pair's gradient histogram before kb_bins counts into it
+__global__ __launch_bounds__(256) void kb_clear_hist(const PairDesc* __restrict__ P) {
+    P[blockIdx.x].ghist[threadIdx.x] = 0u;
 }
 
 // xyz2CloudKernel's gradient magnitude (d2z.cu:235-331) as an 8-bit bin per
@@ -95,7 +99,7 @@ __global__ __launch_bounds__(256) void kb_pyramid(const PairDesc* __restrict__ P
 // (one wave per 4 rows): the tile's histogram is counted in LDS (one LDS
 // atomic per pixel) and merged into the pair's with one global atomic per
 // non-empty bin.
-constexpr int BINS_TILE_H = 16;
+constexpr int BINS_TILE_H = 32;
 __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     const PairDesc& d = P[blockIdx.z];
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * BINS_TILE_H >= d.H) return;
@@ -109,8 +113,14 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     constexpr int HC = 4;
     __shared__ unsigned h[256 * HC];
     __shared__ float gt[TW * TH];
-    __shared__ float cal[96];
     const int W = d.W, H = d.H;
+    // the calibration, read before the kernel's first store: wave-uniform
+    // loads of memory nothing here writes, so they become scalar loads into
+    // scalar registers
+    float cl[96];
+    const RT_GLOBAL float* cg = gmem(d.calib);
+#pragma unroll
+    for (int k = 0; k < 96; k++) cl[k] = cg[k];
     const int lane = lane_id(), wave = threadIdx.x >> 6;
     const int x = blockIdx.x * 64 + lane;
     const int y0 = blockIdx.y * BINS_TILE_H + wave * ROWS;
@@ -142,7 +152,6 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     load_lut(lut);
 #pragma unroll
     for (int k = 0; k < HC; k++) h[threadIdx.x + 256 * k] = 0u;
-    if (threadIdx.x < 96) cal[threadIdx.x] = d.calib[threadIdx.x];
     __syncthreads();
     // words outside the image are 0, and lut[0] = 0: those tile entries are
     // never read (the bounds test keeps every tap 9 pixels inside the image)
@@ -156,47 +165,48 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
         }
     }
     __syncthreads();
-    // the keyframe's pyramid levels 1..L-1 from the tile's core (64 x 16 level-0
-    // pixels: 32 x 8, 16 x 4, 8 x 2), in convert_pyramid_block's order
-    // ((top left + top right) + bottom left) + bottom right, then / 4
+    // the keyframe's pyramid levels 1..L-1 from the tile's core (64 x TH0
+    // level-0 pixels: 32 x TH0/2, 16 x TH0/4, 8 x TH0/8), in
+    // convert_pyramid_block's order ((top left + top right) + bottom left) +
+    // bottom right, then / 4
     if (d.L > 1) {
-        __shared__ float l1s[8][32];
-        __shared__ float l2s[4][16];
+        constexpr int TH0 = BINS_TILE_H;
+        __shared__ float l1s[TH0 / 2][32];
+        __shared__ float l2s[TH0 / 4][16];
         const int W1 = W >> 1, H1 = H >> 1, W2 = W >> 2, H2 = H >> 2;
-        {
-            const int i = threadIdx.x & 31, j = threadIdx.x >> 5;
+#pragma unroll
+        for (int e = threadIdx.x; e < 32 * (TH0 / 2); e += 256) {
+            const int i = e & 31, j = e >> 5;
             const float* q = gt + (2 + 2 * j) * TW + 4 + 2 * i;
             const float v = (q[0] + q[1] + q[TW] + q[TW + 1]) / 4.0f;
             l1s[j][i] = v;
-            const int X1 = blockIdx.x * 32 + i, Y1 = blockIdx.y * (BINS_TILE_H / 2) + j;
+            const int X1 = blockIdx.x * 32 + i, Y1 = blockIdx.y * (TH0 / 2) + j;
             if (X1 < W1 && Y1 < H1) d.gray[0][1][(size_t)Y1 * W1 + X1] = v;
         }
         if (d.L > 2) {
             __syncthreads();
-            if (threadIdx.x < 64) {
+            if (threadIdx.x < 16 * (TH0 / 4)) {
                 const int i = threadIdx.x & 15, j = threadIdx.x >> 4;
                 const float v = (l1s[2 * j][2 * i] + l1s[2 * j][2 * i + 1] + l1s[2 * j + 1][2 * i] + l1s[2 * j + 1][2 * i + 1]) / 4.0f;
                 l2s[j][i] = v;
-                const int X2 = blockIdx.x * 16 + i, Y2 = blockIdx.y * (BINS_TILE_H / 4) + j;
+                const int X2 = blockIdx.x * 16 + i, Y2 = blockIdx.y * (TH0 / 4) + j;
                 if (X2 < W2 && Y2 < H2) d.gray[0][2][(size_t)Y2 * W2 + X2] = v;
             }
             if (d.L > 3) {
                 __syncthreads();
-                if (threadIdx.x < 16) {
+                if (threadIdx.x < 8 * (TH0 / 8)) {
                     const int i = threadIdx.x & 7, j = threadIdx.x >> 3;
                     const float v = (l2s[2 * j][2 * i] + l2s[2 * j][2 * i + 1] + l2s[2 * j + 1][2 * i] + l2s[2 * j + 1][2 * i + 1]) / 4.0f;
-                    const int X3 = blockIdx.x * 8 + i, Y3 = blockIdx.y * (BINS_TILE_H / 8) + j;
+                    const int X3 = blockIdx.x * 8 + i, Y3 = blockIdx.y * (TH0 / 8) + j;
                     if (X3 < (W >> 3) && Y3 < (H >> 3)) d.gray[0][3][(size_t)Y3 * (W >> 3) + X3] = v;
                 }
             }
         }
     }
     const LdsTile tile{gt, tx0, ty0, TW, TH};
-    float cl[96];
-    uniform_calib(cal, cl);
     const U16DepthSrc zs{d.d16[0]};
     const U8Src gs{d.g8[0], lut};
-    const InvFocal inv = inv_focal(cl);
+    const InvFocal inv{d.ifx, d.ify};
     RT_GLOBAL uint8_t* bins = gmem(d.bins);
     auto pixels = [&](auto simple) {
 #pragma unroll
@@ -935,12 +945,13 @@ struct rt_batch {
     int device = 0, max_pairs = 0, max_w = 0, max_h = 0;
     rt_params prm;
     hipStream_t stream = nullptr;
-    // The pose network's proposals run on a stream of their own beside the
-    // keyframe preparation (they read only the frames and write the initial
-    // poses): it waits for the prepare call's uploads and pyramids (inputs),
-    // and the batch stream waits for the proposals (proposed) before tracking.
+    // A second stream beside the keyframe preparation: the current frame's
+    // pyramid (pyramid: its end) and the pose network's proposals (they read
+    // only the frames and write the initial poses; proposed: their end). It
+    // waits for the prepare call's uploads (inputs); the batch stream waits for
+    // both before tracking.
     hipStream_t aux = nullptr;
-    hipEvent_t inputs = nullptr, proposed = nullptr;
+    hipEvent_t inputs = nullptr, proposed = nullptr, pyramid = nullptr;
     int cus = 256;
     std::vector<void*> allocs;
     // per-pair arenas
@@ -1014,7 +1025,8 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
     if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) { delete b; return set_error(RT_ERR_HIP, "stream"); }
     if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->inputs, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->proposed, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&b->proposed, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->pyramid, hipEventDisableTiming) != hipSuccess) {
         rt_batch_destroy(b);
         return set_error(RT_ERR_HIP, "stream");
     }
@@ -1055,6 +1067,7 @@ extern "C" void rt_batch_destroy(rt_batch* b) {
     b->st_desc.release(); b->st_calib.release(); b->st_order.release(); b->st_trel.release();
     if (b->inputs) hipEventDestroy(b->inputs);
     if (b->proposed) hipEventDestroy(b->proposed);
+    if (b->pyramid) hipEventDestroy(b->pyramid);
     if (b->aux) hipStreamDestroy(b->aux);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -1079,8 +1092,8 @@ static BatchView view_of(const rt_batch* b) {
 }
 
 // Keyframe preparation in two parts: the current frame's pyramid (part 1) and
-// the gradient bins with the keyframe's pyramid, selection, points and clouds
-// (part 2); 0: both.
+// the histogram clear, gradient bins with the keyframe's pyramid, selection,
+// points and clouds (part 2); 0: both.
 static void launch_prep(const BatchView& v, hipStream_t s, int part = 0) {
     const int n = v.n;
     if (part != 2) {
@@ -1088,6 +1101,7 @@ static void launch_prep(const BatchView& v, hipStream_t s, int part = 0) {
         kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 1);
     }
     if (part == 1) return;
+    kb_clear_hist<<<n, 256, 0, s>>>(v.desc);
     dim3 gb((v.maxW + 63) / 64, (v.maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
     kb_bins<<<gb, 256, 0, s>>>(v.desc);
     kb_select<<<n, SEL_BS, 0, s>>>(v.desc);
@@ -1297,6 +1311,7 @@ static int build_batch(const rt_params& prm, const BatchMem& m, int n, const rt_
         d.calib = m.calib + 96 * (size_t)i;
         memcpy(&calib[96 * (size_t)i], p.calib, 96 * 4);
         d.simple = calib_simple(p.calib) ? 1 : 0;
+        d.ifx = 1.0f / p.calib[KL]; d.ify = 1.0f / p.calib[KL + 4];
         d.g8[0] = p.ref_gray; d.d16[0] = p.ref_depth; d.g8[1] = p.cur_gray; d.d16[1] = p.cur_depth;
         char* base = m.arena + m.pair_bytes * (size_t)i;
         size_t npx = (size_t)p.width * p.height, off = 0;
@@ -1365,11 +1380,14 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
     if (changed || !b->graph_pyr || !b->graph_prep) {
         if ((rc = capture(b, &b->graph_pyr, 1)) || (rc = capture(b, &b->graph_prep, 2))) return rc;
     }
-    // the pose network's proposals (rt_batch_propose) start after the pyramids:
-    // both read whole frames, while the gradient bins that follow are bound by
-    // arithmetic
-    if (hipGraphLaunch(b->graph_pyr, b->stream) != hipSuccess || hipEventRecord(b->inputs, b->stream) != hipSuccess ||
-        hipGraphLaunch(b->graph_prep, b->stream) != hipSuccess)
+    // The current frame's pyramid runs on the second stream beside the
+    // keyframe's gradient bins (it streams the frame, they are bound by
+    // arithmetic), and the pose network's proposals (rt_batch_propose) follow
+    // it there: both read whole frames. The batch stream waits for the
+    // pyramid before anything after the preparation.
+    if (hipEventRecord(b->inputs, b->stream) != hipSuccess || hipStreamWaitEvent(b->aux, b->inputs, 0) != hipSuccess ||
+        hipGraphLaunch(b->graph_pyr, b->aux) != hipSuccess || hipEventRecord(b->pyramid, b->aux) != hipSuccess ||
+        hipGraphLaunch(b->graph_prep, b->stream) != hipSuccess || hipStreamWaitEvent(b->stream, b->pyramid, 0) != hipSuccess)
         return set_error(RT_ERR_HIP, "graph launch");
     return RT_OK;
 }
@@ -1389,8 +1407,7 @@ extern "C" int rt_batch_propose(rt_batch* b, rt_cnn* net) {
         src[i].g8[0] = d.g8[0]; src[i].g8[1] = d.g8[1];
         src[i].d16[0] = d.d16[0]; src[i].d16[1] = d.d16[1];
     }
-    if (hipStreamWaitEvent(b->aux, b->inputs, 0) != hipSuccess) return set_error(RT_ERR_HIP, "rt_batch_propose: wait");
-    const int rc = cnn_propose(net, b->n, src.data(), b->trel_in, b->aux);
+    const int rc = cnn_propose(net, b->n, src.data(), b->trel_in, b->aux);  // after the pyramid, in stream order
     if (hipEventRecord(b->proposed, b->aux) != hipSuccess || hipStreamWaitEvent(b->stream, b->proposed, 0) != hipSuccess)
         return set_error(RT_ERR_HIP, "rt_batch_propose: join");
     return rc;
@@ -1461,6 +1478,22 @@ extern "C" int rt_batch_results(rt_batch* b, rt_pair_result* out) {
     if (!b || b->n < 1 || !out) return set_error(RT_ERR_STATE, "rt_batch_results: no prepared batch or no output");
     if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
     return fetch_results(b, out);
+}
+
+// rcp_depth(-u) for u = 1 .. n, the reciprocal kb_bins takes of each 16-bit depth
+__global__ __launch_bounds__(256) void k_depth_reciprocals(float* out, int n) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < n) out[i] = rcp_depth(-(float)(i + 1));
+}
+extern "C" int rt_debug_depth_reciprocals(int device, float* host, int n) {
+    if (!host || n < 1 || n > 65535) return set_error(RT_ERR_ARG, "rt_debug_depth_reciprocals: n outside [1, 65535]");
+    if (hipSetDevice(device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
+    float* d = nullptr;
+    if (hipMalloc(&d, 4 * (size_t)n) != hipSuccess) return set_error(RT_ERR_HIP, "rt_debug_depth_reciprocals: alloc");
+    k_depth_reciprocals<<<(n + 255) / 256, 256>>>(d, n);
+    const bool ok = hipGetLastError() == hipSuccess && hipMemcpy(host, d, 4 * (size_t)n, hipMemcpyDeviceToHost) == hipSuccess;
+    hipFree(d);
+    return ok ? RT_OK : set_error(RT_ERR_HIP, "rt_debug_depth_reciprocals: launch or copy");
 }
 
 // ------------------------------------------------------------------ no-alloc batch tracking

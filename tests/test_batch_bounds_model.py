@@ -18,7 +18,7 @@ from recon_amd.tracker import default_params
 
 SIZES = [(160, 120), (320, 240), (480, 360), (640, 480)]
 FIELDS = 40
-BINS_TILE_H = 16
+BINS_TILE_H = 32
 NW_BINS = ((64 + 8) // 4) * (BINS_TILE_H + 4)   # words of the kb_bins tile (TWW * TH)
 
 
@@ -128,8 +128,9 @@ def _kb_pyramid(W, H, L, maxW, maxH):
 
 
 def _kb_bins(W, H, maxW, maxH):
-    """kb_bins: 64 x 16 tiles, lane = column, 4 rows per wave; the clamped
-    depth and gray-word loads, the bin writes and the LDS tile."""
+    """kb_bins: 64 x BINS_TILE_H tiles, lane = column, BINS_TILE_H / 4 rows
+    per wave; the clamped depth and gray-word loads, the bin writes, the LDS
+    tile and the keyframe pyramid levels built from it."""
     bx, by = np.meshgrid(np.arange((maxW + 63) // 64), np.arange((maxH + BINS_TILE_H - 1) // BINS_TILE_H),
                          indexing="ij")
     live = (bx * 64 < W) & (by * BINS_TILE_H < H)
@@ -152,6 +153,18 @@ def _kb_bins(W, H, maxW, maxH):
     # taps of the gradient: in_bounds keeps the projected point in
     # [10, W - 10) x [10, H - 10); the taps are xdi - 1 .. xdi + 2, ydi - 1 .. ydi + 2
     out["gray_taps"] = (H - 11 + 2) * W + (W - 11 + 2)
+    # keyframe pyramid: level-1 value (i, j) averages tile entries rows
+    # 2 + 2j .. 3 + 2j, columns 4 + 2i .. 5 + 2i of the (64 + 8) x (T + 4) tile
+    T = BINS_TILE_H
+    j1, i1 = np.meshgrid(np.arange(T // 2), np.arange(32), indexing="ij")
+    assert (3 + 2 * j1).max() < T + 4 and (5 + 2 * i1).max() < 64 + 8
+    for lq, (nx, ny) in enumerate([(32, T // 2), (16, T // 4), (8, T // 8)], start=1):
+        X = (bx[:, None] * nx + np.arange(nx)[None, :]).ravel()
+        Y = (by[:, None] * ny + np.arange(ny)[None, :]).ravel()
+        Xg, Yg = np.meshgrid(X, Y, indexing="ij")
+        ok = (Xg < (W >> lq)) & (Yg < (H >> lq))
+        if ok.any():
+            out[f"gray_{lq}"] = (Xg[ok] + Yg[ok] * (W >> lq)).max()
     return out
 
 
@@ -225,6 +238,9 @@ def _check_kernels(f, maxes, max_w, max_h, pcs=(0, 5120, 5632, 8192)):
         bins = _kb_bins(W, H, maxW, maxH)
         assert bins["d16"] < W * H and bins["g8_words"] < W * H and bins["bins"] < W * H
         assert bins["gray_taps"] < W * H
+        for k, v in bins.items():
+            if k.startswith("gray_") and k != "gray_taps" and int(k[5:]) < L:
+                assert v < bufsz[k], (p, "kb_bins", k, v)
         sel = _kb_select(W, H, N)
         assert sel["bins"] < W * H and sel["sel"] < N
         ref = _kb_ref_points(W, H, L, N)
