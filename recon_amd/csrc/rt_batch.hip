@@ -106,6 +106,7 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     // tile plus a halo of 2 rows and 4 columns (whole 4-byte words: the
     // width is a multiple of 8, so a word lies entirely inside or outside a row)
     constexpr int TW = 64 + 8, TH = BINS_TILE_H + 4, TWW = TW / 4, NW = TWW * TH;
+    static_assert(TW == 4 * TWW && BINS_TILE_H % 8 == 0, "tile of whole words; three pyramid levels");
     constexpr int ROWS = BINS_TILE_H / 4;  // pixels per thread, one per row
     __shared__ float lut[256];
     // four histogram copies, by lane: lanes with equal bins mostly update
@@ -133,7 +134,7 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     const int xc = min(x, W - 1);
     unsigned zr[ROWS];
 #pragma unroll
-    for (int k = 0; k < ROWS; k++) zr[k] = d16[(size_t)xc + (size_t)min(y0 + k, H - 1) * W];
+    for (int k = 0; k < ROWS; k++) zr[k] = at_offset(d16, 2 * (xc + __umul24(min(y0 + k, H - 1), W)));
     const bool aligned = ((uintptr_t)d.g8[0] & 3) == 0;
     constexpr int NR = (NW + 255) / 256;
     unsigned wv[NR];
@@ -143,8 +144,8 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
         const int e = min((int)threadIdx.x + r * 256, NW - 1);
         const int gx = tx0 + 4 * (e % TWW), gy = ty0 + e / TWW;
         win[r] = gx >= 0 && gx < W && gy >= 0 && gy < H;
-        const size_t o = (size_t)min(max(gx, 0), W - 4) + (size_t)min(max(gy, 0), H - 1) * W;
-        if (aligned) wv[r] = *(const RT_GLOBAL unsigned*)(g8 + o);
+        const unsigned o = (unsigned)min(max(gx, 0), W - 4) + __umul24(min(max(gy, 0), H - 1), W);
+        if (aligned) wv[r] = at_offset((const RT_GLOBAL unsigned*)g8, o);
         else wv[r] = g8[o] | (unsigned)g8[o + 1] << 8 | (unsigned)g8[o + 2] << 16 | (unsigned)g8[o + 3] << 24;
     }
 #pragma unroll
@@ -159,7 +160,7 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     for (int r = 0; r < NR; r++) {
         const int e = threadIdx.x + r * 256;
         if (e < NW) {
-            float* q = gt + (e / TWW) * TW + 4 * (e % TWW);
+            float* q = gt + 4 * e;  // row e / TWW, column 4 (e % TWW): TW = 4 TWW
 #pragma unroll
             for (int j = 0; j < 4; j++) q[j] = lut[(wv[r] >> (8 * j)) & 0xffu];
         }
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
             const float v = (q[0] + q[1] + q[TW] + q[TW + 1]) / 4.0f;
             l1s[j][i] = v;
             const int X1 = blockIdx.x * 32 + i, Y1 = blockIdx.y * (TH0 / 2) + j;
-            if (X1 < W1 && Y1 < H1) d.gray[0][1][(size_t)Y1 * W1 + X1] = v;
+            if (X1 < W1 && Y1 < H1) store_at_offset(gmem(d.gray[0][1]), 4 * ((unsigned)X1 + __umul24(Y1, W1)), v);
         }
         if (d.L > 2) {
             __syncthreads();
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
                 const float v = (l1s[2 * j][2 * i] + l1s[2 * j][2 * i + 1] + l1s[2 * j + 1][2 * i] + l1s[2 * j + 1][2 * i + 1]) / 4.0f;
                 l2s[j][i] = v;
                 const int X2 = blockIdx.x * 16 + i, Y2 = blockIdx.y * (TH0 / 4) + j;
-                if (X2 < W2 && Y2 < H2) d.gray[0][2][(size_t)Y2 * W2 + X2] = v;
+                if (X2 < W2 && Y2 < H2) store_at_offset(gmem(d.gray[0][2]), 4 * ((unsigned)X2 + __umul24(Y2, W2)), v);
             }
             if (d.L > 3) {
                 __syncthreads();
@@ -198,7 +199,7 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
                     const int i = threadIdx.x & 7, j = threadIdx.x >> 3;
                     const float v = (l2s[2 * j][2 * i] + l2s[2 * j][2 * i + 1] + l2s[2 * j + 1][2 * i] + l2s[2 * j + 1][2 * i + 1]) / 4.0f;
                     const int X3 = blockIdx.x * 8 + i, Y3 = blockIdx.y * (TH0 / 8) + j;
-                    if (X3 < (W >> 3) && Y3 < (H >> 3)) d.gray[0][3][(size_t)Y3 * (W >> 3) + X3] = v;
+                    if (X3 < (W >> 3) && Y3 < (H >> 3)) store_at_offset(gmem(d.gray[0][3]), 4 * ((unsigned)X3 + __umul24(Y3, W >> 3)), v);
                 }
             }
         }
@@ -206,16 +207,16 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     const LdsTile tile{gt, tx0, ty0, TW, TH};
     const U16DepthSrc zs{d.d16[0]};
     const U8Src gs{d.g8[0], lut};
-    const InvFocal inv{d.ifx, d.ify};
+    const GradConst gk{d.ifx, d.ify, (float)(W - 10), (float)(H - 10)};
     RT_GLOBAL uint8_t* bins = gmem(d.bins);
     auto pixels = [&](auto simple) {
 #pragma unroll
         for (int k = 0; k < ROWS; k++) {
             const int y = y0 + k;
             if (x < W && y < H) {
-                const float g = grad_mag_tiled<decltype(simple)::value>(-(float)zr[k], zs, gs, tile, W, H, x, y, cl, inv);
+                const float g = grad_mag_tiled<decltype(simple)::value>(-(float)zr[k], zs, gs, tile, W, H, x, y, cl, gk);
                 unsigned b = grad_bin(g);
-                bins[(size_t)x + (size_t)y * W] = (uint8_t)b;
+                store_at_offset(bins, (unsigned)x + __umul24(y, W), (uint8_t)b);
                 atomicAdd(&h[b * HC + (lane & (HC - 1))], 1u);
             }
         }

@@ -54,9 +54,10 @@ __device__ __forceinline__ f2 distort(f2 pu, const float* kc, const float* K) {
     return pd;
 }
 // inBounds (kernelUtils.h:341-344)
-__device__ __forceinline__ bool in_bounds(f2 p, int w, int h) {
-    return p.x >= 10.0f && p.x < (float)(w - 10) && p.y >= 10.0f && p.y < (float)(h - 10);
+__device__ __forceinline__ bool in_bounds(f2 p, float xmax, float ymax) {
+    return p.x >= 10.0f && p.x < xmax && p.y >= 10.0f && p.y < ymax;
 }
+__device__ __forceinline__ bool in_bounds(f2 p, int w, int h) { return in_bounds(p, (float)(w - 10), (float)(h - 10)); }
 // A calibration whose colour camera coincides with the depth camera: TLR and
 // TRL are identities, no radial distortion (k1 = k2 = k5 = 0), KR = KL and a
 // pinhole without skew. Every transform skipped under it is an exact identity
@@ -96,6 +97,16 @@ typedef float nfloat2 __attribute__((ext_vector_type(2)));
 // element i of a device array, loaded through the global address space
 template <class T>
 __device__ __forceinline__ T ldg(const T* p, size_t i) { return gmem(p)[i]; }
+// the element at byte offset off (32 bits) of a device array: one load with a
+// scalar base address and a 32-bit offset per lane (no 64-bit address arithmetic)
+template <class T>
+__device__ __forceinline__ T at_offset(const RT_GLOBAL T* p, unsigned off) {
+    return *(const RT_GLOBAL T*)((const RT_GLOBAL char*)p + off);
+}
+template <class T>
+__device__ __forceinline__ void store_at_offset(RT_GLOBAL T* p, unsigned off, T v) {
+    *(RT_GLOBAL T*)((RT_GLOBAL char*)p + off) = v;
+}
 __device__ __forceinline__ float4 ldg(const float4* p, size_t i) {
     nfloat4 v = ((const RT_GLOBAL nfloat4*)p)[i];
     return make_float4(v.x, v.y, v.z, v.w);
@@ -238,9 +249,10 @@ __device__ __forceinline__ float rcp_depth(float z) {
     const float e = __builtin_fmaf(-z, y, 1.0f);
     return __builtin_fmaf(e, y, y);
 }
-// RN(1/fx), RN(1/fy) of the depth camera (KL) for div_rn
-struct InvFocal { float fx, fy; };
-__device__ __forceinline__ InvFocal inv_focal(const float* c) { return InvFocal{1.0f / c[KL], 1.0f / c[KL + 4]}; }
+// Per-image constants of the gradient pass: RN(1/fx), RN(1/fy) of the depth
+// camera (KL) for div_rn, and the in-bounds limits (float)(W - 10),
+// (float)(H - 10)
+struct GradConst { float ifx, ify, xmax, ymax; };
 
 // A block of gray values staged in LDS: image pixels [x0, x0+w) x [y0, y0+h),
 // row pitch w, holding exactly the values the image source returns.
@@ -265,11 +277,11 @@ struct LdsTile {
 // onto the next integer, and only xe and ys are tested.
 template <bool SIMPLE, class ZS, class GS>
 __device__ __forceinline__ float grad_mag_tiled(float zc, const ZS& z, const GS& g0, const LdsTile& tile, int W,
-                                                int H, int xi, int yi, const float* c, InvFocal inv) {
+                                                int H, int xi, int yi, const float* c, GradConst k) {
     f3 p3{0.0f, 0.0f, 0.0f};
     if (zc < -c[MIND] && zc > -c[MAXD]) {  // backproject
-        p3.x = div_rn(((float)xi - c[KL + 2]) * zc, c[KL], inv.fx);
-        p3.y = div_rn(((float)yi - c[KL + 5]) * zc, c[KL + 4], inv.fy);
+        p3.x = div_rn(((float)xi - c[KL + 2]) * zc, c[KL], k.ifx);
+        p3.y = div_rn(((float)yi - c[KL + 5]) * zc, c[KL + 4], k.ify);
         p3.z = zc;
     }
     if (p3.z < 0.0f) {
@@ -277,12 +289,12 @@ __device__ __forceinline__ float grad_mag_tiled(float zc, const ZS& z, const GS&
         const float iz = SIMPLE ? rcp_depth(r3.z) : 1.0f / r3.z;
         f2 pu{div_rn(r3.x, r3.z, iz), div_rn(r3.y, r3.z, iz)};
         f2 p = SIMPLE ? project_simple(pu, c + KR) : distort(pu, c + KCR, c + KR);
-        if (in_bounds(p, W, H)) {
+        if (in_bounds(p, k.xmax, k.ymax)) {
             int xdi = (int)p.x, ydi = (int)p.y;
             float fx = p.x - (float)xdi, fy = p.y - (float)ydi;
             int xe = (int)(p.x + 1.0f), ys = (int)(p.y + 1.0f);
             if (xe == xdi + 1 && ys == ydi + 1 && tile.covers(xdi - 1, ydi - 1, xdi + 2, ydi + 2)) {
-                const float* q = tile.p + (ydi - tile.y0) * tile.w + (xdi - tile.x0);
+                const float* q = tile.p + __umul24(ydi - tile.y0, tile.w) + (xdi - tile.x0);  // both >= 1: covered
                 const int P = tile.w;
                 float a01 = q[-P], a02 = q[-P + 1];
                 float a10 = q[-1], a11 = q[0], a12 = q[1], a13 = q[2];
