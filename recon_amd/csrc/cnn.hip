@@ -26,21 +26,10 @@
 
 namespace rt {
 
-typedef uint16_t bf16;
 typedef short s4 __attribute__((ext_vector_type(4)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(bf16 v) { return __uint_as_float((uint32_t)v << 16); }
-// round to nearest even; NaN stays quiet NaN
-__device__ __forceinline__ bf16 f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7f800000u) == 0x7f800000u) {
-        if (u & 0x7fffffu) u |= 0x400000u;
-        return (bf16)(u >> 16);
-    }
-    u += 0x7fffu + ((u >> 16) & 1u);
-    return (bf16)(u >> 16);
-}
 __device__ __forceinline__ bool bf_pos(bf16 v) { return !(v & 0x8000u) && v != 0; }
 
 struct ConvGeom {
@@ -843,10 +832,13 @@ __global__ void k_pack(const float* __restrict__ W, int Co, int Ci, int Kp, bf16
 }
 
 // ---------------------------------------------------------------- inputs and proposals
-// The network's inputs, one cell of (W / IW) x (H / IH) pixels per thread and
-// per frame: mean gray / 255 - 0.5 and the mean inverse depth (1000 / mm, by
-// the hardware reciprocal: within 1 ulp, far below the bfloat16 rounding of
-// the result) of the valid pixels - 1, or -1 without valid depth.
+// The network's inputs, one cell of bw x bh = (W / IW) x (H / IH) pixels per
+// thread and per frame (cell_features; the inverse depths by the hardware
+// reciprocal: within 1 ulp, far below the bfloat16 rounding of the result).
+// The inverse depths are summed per column over the rows in order; the
+// column sums are merged pairwise by a stack (two sums of equal size merge,
+// left + right; what remains at the end is added right to left), which for
+// a power-of-two bw is the balanced tree ((c0 + c1) + (c2 + c3)) + ...
 __global__ __launch_bounds__(256) void k_inputs(const FrameSrc* __restrict__ src, int IW, int IH, bf16* __restrict__ x) {
     const int n = blockIdx.y;
     const int cell = blockIdx.x * 256 + threadIdx.x;
@@ -854,40 +846,34 @@ __global__ __launch_bounds__(256) void k_inputs(const FrameSrc* __restrict__ src
     const FrameSrc s = src[n];
     const int bw = s.W / IW, bh = s.H / IH;
     const int cx = cell % IW, cy = cell / IW;
-    float feat[4];
+    bf16* o = x + ((size_t)n * IW * IH + cell) * 4;
     for (int f = 0; f < 2; f++) {
-        unsigned gs = 0;
-        float ids = 0.0f;
-        int nv = 0;
         const uint8_t* g8 = s.g8[f];
         const uint16_t* d16 = s.d16[f];
-        for (int yy = 0; yy < bh; yy++) {
-            size_t row = (size_t)(cy * bh + yy) * s.W + (size_t)cx * bw;
-            if (bw == 8 && !(((uintptr_t)(g8 + row)) & 7) && !(((uintptr_t)(d16 + row)) & 15)) {
-                // one 8-byte gray and one 16-byte depth load per row, same order of sums
-                uint2 gq = *(const uint2*)(g8 + row);
-                uint4 dq = *(const uint4*)(d16 + row);
-                const unsigned gw[2] = {gq.x, gq.y};
-                const unsigned dw[4] = {dq.x, dq.y, dq.z, dq.w};
-#pragma unroll
-                for (int xx = 0; xx < 8; xx++) {
-                    gs += (gw[xx >> 2] >> (8 * (xx & 3))) & 0xffu;
-                    unsigned d = (dw[xx >> 1] >> (16 * (xx & 1))) & 0xffffu;
-                    if (d) { ids += 1000.0f * __builtin_amdgcn_rcpf((float)d); nv++; }
-                }
-            } else {
-                for (int xx = 0; xx < bw; xx++) {
-                    gs += g8[row + xx];
-                    unsigned d = d16[row + xx];
-                    if (d) { ids += 1000.0f * __builtin_amdgcn_rcpf((float)d); nv++; }
-                }
-            }
+        const size_t row0 = (size_t)(cy * bh) * s.W + (size_t)cx * bw;
+        if (bw == 8 && bh == 8 && !(((uintptr_t)(g8 + row0)) & 7) && !(((uintptr_t)(d16 + row0)) & 15) && !(s.W & 7)) {
+            cell8_features(g8, d16, s.W, cx * 8, cy * 8, o + 2 * f);
+            continue;
         }
-        feat[2 * f] = (float)gs / (255.0f * (float)(bw * bh)) - 0.5f;
-        feat[2 * f + 1] = nv ? ids / (float)nv - 1.0f : -1.0f;
+        unsigned gs = 0, nv = 0;
+        float sv[16];  // bw < 65536 columns: at most 16 pending sums
+        int sl[16], sp = 0;
+        for (int xx = 0; xx < bw; xx++) {
+            float v = 0.0f;
+            for (int yy = 0; yy < bh; yy++) {
+                const size_t i = row0 + (size_t)yy * s.W + xx;
+                gs += g8[i];
+                unsigned d = d16[i];
+                if (d) { v += inv_depth_m(d); nv++; }
+            }
+            int l = 0;
+            while (sp > 0 && sl[sp - 1] == l) { v = sv[--sp] + v; l++; }
+            sv[sp] = v; sl[sp] = l; sp++;
+        }
+        float ids = sv[sp - 1];
+        for (int k = sp - 2; k >= 0; k--) ids = sv[k] + ids;
+        cell_features(gs, ids, nv, bw * bh, o + 2 * f);
     }
-    bf16* o = x + ((size_t)n * IW * IH + cell) * 4;
-    for (int c = 0; c < 4; c++) o[c] = f2bf(feat[c]);
 }
 
 // Training-set symmetries of a pinhole camera with a centred principal point.
@@ -1348,13 +1334,20 @@ extern "C" int rt_cnn_make_inputs(rt_cnn* net, int n, const rt_pair* pairs, uint
 }
 
 namespace rt {
-int cnn_propose(rt_cnn* net, int n, const FrameSrc* src, float* trel_dev, hipStream_t s) {
+bool cnn_cells8(const rt_cnn* net, int W, int H) {
+    return net && net->cfg.in_width * 8 == W && net->cfg.in_height * 8 == H;
+}
+
+int cnn_propose(rt_cnn* net, int n, const FrameSrc* src, float* trel_dev, hipStream_t s, const bf16* x_ready) {
     if (!net || n < 1 || n > net->max_batch) return set_error(RT_ERR_ARG, "rt_batch_propose: batch larger than the network's max_batch");
     if (!net->packed) return set_error(RT_ERR_STATE, "rt_batch_propose: network parameters not set");
     if (hipSetDevice(net->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
-    bf16* x = net->xin;
-    int rc = make_inputs(net, n, src, x, s);
-    if (rc) return rc;
+    const bf16* x = x_ready;
+    int rc = 0;
+    if (!x) {
+        if ((rc = make_inputs(net, n, src, net->xin, s))) return rc;
+        x = net->xin;
+    }
     if ((rc = forward(net, n, x, s))) return rc;
     const int C6 = net->cfg.channels[6], P6 = net->H[6] * net->W[6];
     k_head<<<n, 256, 2 * C6 * 4, s>>>(P6, C6, net->act[6], net->params + net->fc_w, net->params + net->fc_b, nullptr,

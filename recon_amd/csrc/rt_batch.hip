@@ -38,6 +38,7 @@ struct PairDesc {
     const float* calib;
     int simple;             // calib_simple(calib), evaluated at preparation
     float ifx, ify;         // RN(1 / fx), RN(1 / fy) of the depth camera (IEEE divisions on the host)
+    bf16* feat;             // the pose network's inputs of the pair (cells of 8 x 8 pixels, 4 channels), or null
     const uint8_t* g8[2];
     const uint16_t* d16[2];
     float* gray[2][4];
@@ -83,11 +84,50 @@ __device__ __forceinline__ void load_lut(float* lut) {
 
 // pyramid levels 1..L-1 of one frame (fastDownSampleHdrKernel, downSample2Cuda.cu:35-42);
 // the keyframe's (which 0) are built by kb_bins from its gray tile
+// With d.feat set (which 1), also the pose network's inputs of the current
+// frame over the block's 8 x 8 cells: four threads per cell, two columns
+// each, column sums over the rows in order, then pairwise: the order of
+// cell8_features, the network's input builder
 __global__ __launch_bounds__(256) void kb_pyramid(const PairDesc* __restrict__ P, int which) {
     const PairDesc& d = P[blockIdx.z];
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 64 >= d.H) return;
     convert_pyramid_block<3>(blockIdx.x, blockIdx.y, d.g8[which], nullptr, nullptr, d.W, d.H, d.L, d.calib, nullptr,
                              d.gray[which][1], d.gray[which][2], d.gray[which][3], nullptr);
+    bf16* const feat = d.feat;
+    if (which != 1 || !feat) return;
+    const int W = d.W, H = d.H;
+    const int cell = threadIdx.x >> 2, part = threadIdx.x & 3;
+    const int cx = blockIdx.x * 8 + (cell & 7), cy = blockIdx.y * 8 + (cell >> 3);
+    const bool valid = cx < (W >> 3) && cy < (H >> 3);
+    // clamped into the image; what an invalid cell sums is never written
+    const unsigned px = min(cx * 8 + 2 * part, W - 2), py = min(cy * 8, H - 8);
+    const RT_GLOBAL uint8_t* g8 = gmem(d.g8[1]);
+    const RT_GLOBAL uint16_t* d16 = gmem(d.d16[1]);
+    unsigned gw[8], dw[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        const unsigned o = px + __umul24(py + r, W);
+        gw[r] = at_offset((const RT_GLOBAL uint16_t*)g8, o);    // two gray bytes (o even)
+        dw[r] = at_offset((const RT_GLOBAL unsigned*)d16, 2 * o);  // two depths
+    }
+    unsigned fg = 0, fn = 0;
+    float c0 = 0.0f, c1 = 0.0f;
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+        fg += (gw[r] & 0xffu) + (gw[r] >> 8);
+        const unsigned z0 = dw[r] & 0xffffu, z1 = dw[r] >> 16;
+        if (z0) { c0 += inv_depth_m(z0); fn++; }
+        if (z1) { c1 += inv_depth_m(z1); fn++; }
+    }
+    // thread part p holds columns 2p, 2p + 1: (c0 + c1) + (c2 + c3), then + (c4 + c5) + (c6 + c7)
+    float fi = c0 + c1;
+#pragma unroll
+    for (int o = 1; o < 4; o <<= 1) {
+        fi += __shfl_xor(fi, o, 64);
+        fg += __shfl_xor(fg, o, 64);
+        fn += __shfl_xor(fn, o, 64);
+    }
+    if (part == 0 && valid) cell_features(fg, fi, fn, 64, feat + 4 * ((size_t)cy * (W >> 3) + cx) + 2);
 }
 // clears each pair's gradient histogram before kb_bins counts into it
 __global__ __launch_bounds__(256) void kb_clear_hist(const PairDesc* __restrict__ P) {
@@ -135,6 +175,16 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     unsigned zr[ROWS];
 #pragma unroll
     for (int k = 0; k < ROWS; k++) zr[k] = at_offset(d16, 2 * (xc + __umul24(min(y0 + k, H - 1), W)));
+    // With d.feat set (W and H multiples of 8), the pose network's keyframe
+    // inputs too: each wave's 8 rows x 64 columns are 8 cells of 8 x 8 pixels,
+    // the lane's column sums taken over its rows in order (cell_features)
+    static_assert(ROWS == 8, "one cell row per wave");
+    bf16* const feat = d.feat;
+    unsigned gr[ROWS];
+    if (feat) {
+#pragma unroll
+        for (int k = 0; k < ROWS; k++) gr[k] = at_offset(g8, xc + __umul24(min(y0 + k, H - 1), W));
+    }
     const bool aligned = ((uintptr_t)d.g8[0] & 3) == 0;
     constexpr int NR = (NW + 255) / 256;
     unsigned wv[NR];
@@ -223,6 +273,24 @@ __global__ __launch_bounds__(256) void kb_bins(const PairDesc* __restrict__ P) {
     };
     if (d.simple) pixels(std::true_type{});
     else pixels(std::false_type{});
+    if (feat) {
+        unsigned fg = 0, fn = 0;
+        float fi = 0.0f;
+#pragma unroll
+        for (int k = 0; k < ROWS; k++) {
+            fg += gr[k];
+            if (zr[k]) { fi += inv_depth_m(zr[k]); fn++; }
+        }
+        // columns 8c .. 8c+7 of a cell in lanes 8c .. 8c+7: pairwise over lanes
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            fi += __shfl_xor(fi, o, 64);
+            fg += __shfl_xor(fg, o, 64);
+            fn += __shfl_xor(fn, o, 64);
+        }
+        if ((lane & 7) == 0 && x < W && y0 < H)
+            cell_features(fg, fi, fn, 64, feat + 4 * ((size_t)(y0 >> 3) * (W >> 3) + (x >> 3)));
+    }
     __syncthreads();
     unsigned v = 0u;
 #pragma unroll
@@ -949,10 +1017,17 @@ struct rt_batch {
     // A second stream beside the keyframe preparation: the current frame's
     // pyramid (pyramid: its end) and the pose network's proposals (they read
     // only the frames and write the initial poses; proposed: their end). It
-    // waits for the prepare call's uploads (inputs); the batch stream waits for
-    // both before tracking.
+    // waits for the prepare call's uploads (inputs), and the proposals for the
+    // end of the gradient bins (binned); the batch stream waits for both
+    // before tracking.
     hipStream_t aux = nullptr;
-    hipEvent_t inputs = nullptr, proposed = nullptr, pyramid = nullptr;
+    hipEvent_t inputs = nullptr, proposed = nullptr, pyramid = nullptr, binned = nullptr;
+    // The pose network's inputs, built by the preparation from the frames it
+    // reads anyway once rt_batch_propose has been called on this batch
+    // (feat_on): max_pairs x (max_w / 8) x (max_h / 8) x 4 bfloat16;
+    // feat_ready: the last preparation built them for every pair.
+    bf16* feat = nullptr;
+    bool feat_on = false, feat_ready = false;
     int cus = 256;
     std::vector<void*> allocs;
     // per-pair arenas
@@ -976,7 +1051,7 @@ struct rt_batch {
     // tuning and graphs
     std::map<std::string, std::pair<int, int>> tuned;  // key -> (variant, grid)
     int variant = -1, grid = 0;
-    hipGraphExec_t graph_pyr = nullptr, graph_prep = nullptr, graph_track = nullptr;
+    hipGraphExec_t graph_pyr = nullptr, graph_bins = nullptr, graph_sel = nullptr, graph_track = nullptr;
     int graph_n = -1, graph_variant = -1, graph_grid = -1, graph_maxN = -1;  // the track graph's launch shape
 };
 
@@ -1027,7 +1102,8 @@ extern "C" int rt_batch_create(int device, int max_pairs, int max_w, int max_h, 
     if (hipStreamCreateWithFlags(&b->aux, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&b->inputs, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&b->proposed, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&b->pyramid, hipEventDisableTiming) != hipSuccess) {
+        hipEventCreateWithFlags(&b->pyramid, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->binned, hipEventDisableTiming) != hipSuccess) {
         rt_batch_destroy(b);
         return set_error(RT_ERR_HIP, "stream");
     }
@@ -1062,13 +1138,16 @@ extern "C" void rt_batch_destroy(rt_batch* b) {
     if (b->aux) hipStreamSynchronize(b->aux);
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->graph_pyr) hipGraphExecDestroy(b->graph_pyr);
-    if (b->graph_prep) hipGraphExecDestroy(b->graph_prep);
+    if (b->graph_bins) hipGraphExecDestroy(b->graph_bins);
+    if (b->graph_sel) hipGraphExecDestroy(b->graph_sel);
     if (b->graph_track) hipGraphExecDestroy(b->graph_track);
     for (void* p : b->allocs) hipFree(p);
     b->st_desc.release(); b->st_calib.release(); b->st_order.release(); b->st_trel.release();
     if (b->inputs) hipEventDestroy(b->inputs);
     if (b->proposed) hipEventDestroy(b->proposed);
     if (b->pyramid) hipEventDestroy(b->pyramid);
+    if (b->binned) hipEventDestroy(b->binned);
+    if (b->feat) hipFree(b->feat);
     if (b->aux) hipStreamDestroy(b->aux);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
@@ -1092,19 +1171,22 @@ static BatchView view_of(const rt_batch* b) {
                      b->prm.solver, b->prm.robust, b->prm.mad_bins, b->prm.mode, b->prm.lambda};
 }
 
-// Keyframe preparation in two parts: the current frame's pyramid (part 1) and
-// the histogram clear, gradient bins with the keyframe's pyramid, selection,
-// points and clouds (part 2); 0: both.
+// Keyframe preparation in three parts: the current frame's pyramid (part 1),
+// the histogram clear and the gradient bins with the keyframe's pyramid (part
+// 2), the selection, points and clouds (part 3); 0: all.
 static void launch_prep(const BatchView& v, hipStream_t s, int part = 0) {
     const int n = v.n;
-    if (part != 2) {
+    if (part == 0 || part == 1) {
         dim3 g64((v.maxW + 63) / 64, (v.maxH + 63) / 64, n);
         kb_pyramid<<<g64, 256, 0, s>>>(v.desc, 1);
     }
     if (part == 1) return;
-    kb_clear_hist<<<n, 256, 0, s>>>(v.desc);
-    dim3 gb((v.maxW + 63) / 64, (v.maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
-    kb_bins<<<gb, 256, 0, s>>>(v.desc);
+    if (part != 3) {
+        kb_clear_hist<<<n, 256, 0, s>>>(v.desc);
+        dim3 gb((v.maxW + 63) / 64, (v.maxH + BINS_TILE_H - 1) / BINS_TILE_H, n);
+        kb_bins<<<gb, 256, 0, s>>>(v.desc);
+    }
+    if (part == 2) return;
     kb_select<<<n, SEL_BS, 0, s>>>(v.desc);
     kb_ref_points<<<dim3((v.maxN + 255) / 256, n), 256, 0, s>>>(v.desc);
     if (v.mode != RT_PHOTOMETRIC) {
@@ -1371,6 +1453,15 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
     int maxW = 0, maxH = 0, maxN = 0;
     int rc = build_batch(b->prm, m, n, pairs, desc.data(), calib.data(), trel.data(), order.data(), maxW, maxH, maxN);
     if (rc) return rc;
+    // the network's inputs in the preparation: pairs of one size, a multiple
+    // of 8, with the current frame 2-byte (gray) and 4-byte (depth) aligned
+    bool feat = b->feat_on;
+    for (int i = 0; i < n && feat; i++)
+        feat = desc[i].W == desc[0].W && desc[i].H == desc[0].H && desc[i].W % 8 == 0 && desc[i].H % 8 == 0 &&
+               !((uintptr_t)desc[i].g8[1] & 1) && !((uintptr_t)desc[i].d16[1] & 3);
+    for (int i = 0; i < n; i++)
+        desc[i].feat = feat ? b->feat + (size_t)i * (desc[i].W / 8) * (desc[i].H / 8) * 4 : nullptr;
+    b->feat_ready = feat;
     if (!b->st_desc.upload(b->desc_dev, desc.data(), sizeof(PairDesc) * n, b->stream) ||
         !b->st_calib.upload(b->calib_dev, calib.data(), calib.size() * 4, b->stream) ||
         !b->st_order.upload(b->order_dev, order.data(), 4 * (size_t)n, b->stream) ||
@@ -1378,17 +1469,20 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
         return set_error(RT_ERR_HIP, "descriptor upload");
     bool changed = b->n != n || b->maxW != maxW || b->maxH != maxH || b->maxN != maxN;
     b->n = n; b->desc = desc; b->maxW = maxW; b->maxH = maxH; b->maxN = maxN;
-    if (changed || !b->graph_pyr || !b->graph_prep) {
-        if ((rc = capture(b, &b->graph_pyr, 1)) || (rc = capture(b, &b->graph_prep, 2))) return rc;
+    if (changed || !b->graph_pyr || !b->graph_bins || !b->graph_sel) {
+        if ((rc = capture(b, &b->graph_pyr, 1)) || (rc = capture(b, &b->graph_bins, 2)) || (rc = capture(b, &b->graph_sel, 3)))
+            return rc;
     }
     // The current frame's pyramid runs on the second stream beside the
     // keyframe's gradient bins (it streams the frame, they are bound by
-    // arithmetic), and the pose network's proposals (rt_batch_propose) follow
-    // it there: both read whole frames. The batch stream waits for the
-    // pyramid before anything after the preparation.
+    // arithmetic). The pose network's proposals (rt_batch_propose) follow it
+    // there once the bins are done (binned), beside the selection and the
+    // reference points, which are bound by latency. The batch stream waits for
+    // the pyramid before anything after the preparation.
     if (hipEventRecord(b->inputs, b->stream) != hipSuccess || hipStreamWaitEvent(b->aux, b->inputs, 0) != hipSuccess ||
         hipGraphLaunch(b->graph_pyr, b->aux) != hipSuccess || hipEventRecord(b->pyramid, b->aux) != hipSuccess ||
-        hipGraphLaunch(b->graph_prep, b->stream) != hipSuccess || hipStreamWaitEvent(b->stream, b->pyramid, 0) != hipSuccess)
+        hipGraphLaunch(b->graph_bins, b->stream) != hipSuccess || hipEventRecord(b->binned, b->stream) != hipSuccess ||
+        hipGraphLaunch(b->graph_sel, b->stream) != hipSuccess || hipStreamWaitEvent(b->stream, b->pyramid, 0) != hipSuccess)
         return set_error(RT_ERR_HIP, "graph launch");
     return RT_OK;
 }
@@ -1408,7 +1502,18 @@ extern "C" int rt_batch_propose(rt_batch* b, rt_cnn* net) {
         src[i].g8[0] = d.g8[0]; src[i].g8[1] = d.g8[1];
         src[i].d16[0] = d.d16[0]; src[i].d16[1] = d.d16[1];
     }
-    const int rc = cnn_propose(net, b->n, src.data(), b->trel_in, b->aux);  // after the pyramid, in stream order
+    // inputs built by the preparation when it built them for this network's
+    // input size; else the network builds them (the same values)
+    const bool ready = b->feat_ready && cnn_cells8(net, b->desc[0].W, b->desc[0].H);
+    if (!b->feat_on) {
+        // from the next preparation on: one buffer for the batch's lifetime
+        const size_t bytes = (size_t)b->max_pairs * (b->max_w / 8) * (b->max_h / 8) * 4 * sizeof(bf16);
+        if (hipMalloc(&b->feat, bytes) != hipSuccess) return set_error(RT_ERR_HIP, "rt_batch_propose: input buffer");
+        b->feat_on = true;
+    }
+    if (hipStreamWaitEvent(b->aux, b->binned, 0) != hipSuccess) return set_error(RT_ERR_HIP, "rt_batch_propose: wait");
+    // after the pyramid and the bins
+    const int rc = cnn_propose(net, b->n, src.data(), b->trel_in, b->aux, ready ? b->feat : nullptr);
     if (hipEventRecord(b->proposed, b->aux) != hipSuccess || hipStreamWaitEvent(b->stream, b->proposed, 0) != hipSuccess)
         return set_error(RT_ERR_HIP, "rt_batch_propose: join");
     return rc;

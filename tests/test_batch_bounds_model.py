@@ -124,6 +124,18 @@ def _kb_pyramid(W, H, L, maxW, maxH):
         ev = ((t & 15)[inside] % 2 == 0) & ((t >> 4)[inside] % 2 == 0)
         out["gray_3"] = ((y0[ev] // 8) * (W >> 3) + x0[ev] // 8).max()
         assert ((t & 15)[inside][ev] + 1).max() <= 15 and ((t >> 4)[inside][ev] + 1).max() <= 15  # l2tile[16][17]
+    if W % 8 == 0 and H % 8 == 0:
+        # the network's inputs of the current frame: 4 threads per 8x8 cell, two
+        # columns each, loads clamped to px <= W - 2, py <= H - 8; rows py .. py + 7
+        cell, part = t[live] >> 2, t[live] & 3
+        cx = bx[live] * 8 + (cell & 7)
+        cy = by[live] * 8 + (cell >> 3)
+        px = np.minimum(cx * 8 + 2 * part, W - 2)
+        py = np.minimum(cy * 8, H - 8)
+        out["g8_feat"] = ((py + 7) * W + px + 1).max()                # two gray bytes
+        out["d16_feat"] = ((py + 7) * W + px + 1).max()               # two depths
+        ok = (cx < W // 8) & (cy < H // 8)
+        out["feat"] = (4 * (cy[ok] * (W // 8) + cx[ok]) + 3).max()    # channels 2, 3
     return out
 
 
@@ -232,6 +244,7 @@ def _check_kernels(f, maxes, max_w, max_h, pcs=(0, 5120, 5632, 8192)):
         bufsz = {"g8": W * H, "d16": W * H, "bins": W * H, "sel": N, "pts": N}
         for l in range(1, L):
             bufsz[f"gray_{l}"] = (W >> l) * (H >> l)
+        bufsz.update(g8_feat=W * H, d16_feat=W * H, feat=4 * (W // 8) * (H // 8))
         pyr = _kb_pyramid(W, H, L, maxW, maxH)
         for k, v in pyr.items():
             assert v < bufsz[k], (p, "kb_pyramid", k, v)

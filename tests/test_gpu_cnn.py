@@ -176,6 +176,51 @@ def test_batch_proposals_are_the_network_outputs():
     assert np.allclose(Trel, expect, atol=2e-6), np.abs(Trel - expect).max()
 
 
+def test_batch_inputs_built_in_preparation():
+    """Once rt_batch_propose has been called on a batch, the preparation
+    builds the network's inputs itself (the keyframe's in the gradient-bin
+    pass, the current frame's in its pyramid pass) instead of the network's
+    input builder; the proposals must not change by a bit. 640x480 pairs
+    (80x60 cells of 8x8), with depth-less blocks in either frame."""
+    print("case cnn inputs built in the preparation", flush=True)
+    from recon_amd import PHOTOMETRIC, synth
+    from recon_amd.tracker import BatchTracker
+    W, H, n = 640, 480, 3
+    frames = [list(synth.make_pair(30 + i, W, H, 8.0, 120.0)[:4]) for i in range(n)]
+    frames[0][1] = frames[0][1].copy(); frames[0][1][:24, :40] = 0     # keyframe depth
+    frames[1][3] = frames[1][3].copy(); frames[1][3][100:117, 300:333] = 0  # current depth
+    bufs = [[DeviceBuffer.from_array(a) for a in f] for f in frames]
+    cal = synth.calib_block_np(W, H)
+    pairs = [dict(ref_gray=b[0].ptr, ref_depth=b[1].ptr, cur_gray=b[2].ptr, cur_depth=b[3].ptr, width=W, height=H,
+                  calib=cal) for b in bufs]
+    cfg = default_config()
+    assert cfg.in_width * 8 == W and cfg.in_height * 8 == H
+    net = PoseNet(8, config=cfg)
+    p = init_params(cfg, 6)
+    layers, fc_w, fc_b, total = layout(cfg)
+    p[fc_w:total] = np.random.default_rng(2).normal(0, 0.5, total - fc_w)
+    net.set_params(p)
+    x = DeviceBuffer(2 * net.input_elems * n)
+    o = DeviceBuffer(24 * n)
+    net.make_inputs(pair_array(pairs), n, x.ptr)
+    net.forward(n, x.ptr, o.ptr)
+    from recon_amd.hip import hip
+    hip().hipDeviceSynchronize()
+    expect = target_to_pose(cfg, o.to_array(np.float32, 6 * n))
+    bt = BatchTracker(n, W, H, mode=PHOTOMETRIC, n_select=2048, iters=(0, 0, 0, 0))
+    prep = bt.prepare(pairs)
+    got = []
+    for step in range(3):  # the network's input builder, then the preparation's inputs twice
+        bt.prepare_only(prep)
+        bt.propose(net)
+        Trel, _, _, st = bt.retrack(None, n)
+        assert np.all(st == 0)
+        got.append(Trel.copy())
+    assert np.allclose(got[0], expect, atol=2e-6), np.abs(got[0] - expect).max()
+    for g in got[1:]:
+        assert np.array_equal(g, got[0]), np.abs(g - got[0]).max()
+
+
 @pytest.mark.parametrize("flag", range(8))
 def test_augmentation_is_the_camera_symmetry(flag):
     """Mirroring the columns (rows) of both frames mirrors camera x (y):
