@@ -355,8 +355,9 @@ def main():
                "unit": "TFLOP/s", "frac": round(flops / (f_ms * 1e-3) / 1e12 / BF16_PEAK_TFLOPS, 5),
                "ms": round(f_ms, 3), "flops_per_pair": CNN_FLOPS_PER_PAIR, "proposal_ms": round(c_ms, 3),
                "note": "ms: the six convolutions + head of the batch (forward on the network's stream); "
-                       "proposal_ms: the whole proposal step (inputs from the raw frames, which stream 7.5 GB, "
-                       "+ forward + poses)"}
+                       "proposal_ms: the whole proposal step on the batch (forward + poses, on the inputs "
+                       "the batch preparation built; the network's own input builder, used before the "
+                       "first proposal of a batch, streams both frames, 7.5 GB)"}
         del xin, yout
 
     # ---- two batches in flight (learned): the host's preparation of step k+1
