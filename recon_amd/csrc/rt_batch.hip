@@ -540,14 +540,21 @@ __global__ __launch_bounds__(SEL_BS, RT_SEL_WPE) void kb_select(const PairDesc* 
     // a short selection leaves zeros behind the selected indices
     for (int k = (total > 0 ? total : 0) + t; k < N; k += SEL_BS) sel[k] = 0;
 }
-__global__ __launch_bounds__(256) void kb_ref_points(const PairDesc* __restrict__ P) {
-    const PairDesc& d = P[blockIdx.y];
+// XCD-aware: workgroup b runs on XCD b % 8 (dispatch is round-robin), and
+// all chunks of pair p get workgroups with b % 8 == p % 8, so that the taps a
+// pair's points share are fetched into one L2 instead of eight. Grid:
+// 8 x ceil(n / 8) x chunks workgroups.
+__global__ __launch_bounds__(256) void kb_ref_points(const PairDesc* __restrict__ P, int n, int chunks) {
+    const int b = blockIdx.x, k = b >> 3;
+    const int pair = (b & 7) + 8 * (k / chunks), chunk = k % chunks;
+    if (pair >= n) return;
+    const PairDesc& d = P[pair];
     __shared__ float lut[256];
     __shared__ float cal[96];
     load_lut(lut);
     if (threadIdx.x < 96) cal[threadIdx.x] = d.calib[threadIdx.x];
     __syncthreads();
-    int i = blockIdx.x * 256 + threadIdx.x;
+    int i = chunk * 256 + threadIdx.x;
     if (i >= d.N) return;
     float cl[96];
     uniform_calib(cal, cl);
@@ -1188,7 +1195,8 @@ static void launch_prep(const BatchView& v, hipStream_t s, int part = 0) {
     }
     if (part == 2) return;
     kb_select<<<n, SEL_BS, 0, s>>>(v.desc);
-    kb_ref_points<<<dim3((v.maxN + 255) / 256, n), 256, 0, s>>>(v.desc);
+    const int chunks = (v.maxN + 255) / 256;
+    kb_ref_points<<<8 * ((n + 7) / 8) * chunks, 256, 0, s>>>(v.desc, n, chunks);
     if (v.mode != RT_PHOTOMETRIC) {
         kb_clouds<<<dim3((v.maxW + 15) / 16, (v.maxH + 15) / 16, n), 256, 0, s>>>(v.desc);
         for (int l = 1; l < 4; l++)

@@ -192,6 +192,24 @@ def _kb_select(W, H, N):
     return {"bins": last, "sel": N - 1}
 
 
+def _ref_points_blocks(n, chunks):
+    """kb_ref_points' XCD-aware grid: workgroup b -> (pair, chunk), pairs of
+    one XCD (b % 8) together; returns the (pair, chunk) of every live block."""
+    b = np.arange(8 * ((n + 7) // 8) * chunks)
+    k = b >> 3
+    pair, chunk = (b & 7) + 8 * (k // chunks), k % chunks
+    assert np.all(pair % 8 == b % 8)
+    live = pair < n
+    return pair[live], chunk[live]
+
+
+@pytest.mark.parametrize("n,chunks", [(1, 1), (3, 32), (8, 32), (13, 7), (4096, 32), (257, 5)])
+def test_ref_points_grid_covers_each_chunk_once(n, chunks):
+    pair, chunk = _ref_points_blocks(n, chunks)
+    key = pair * chunks + chunk
+    assert key.size == n * chunks and np.array_equal(np.sort(key), np.arange(n * chunks))
+
+
 def _kb_ref_points(W, H, L, N):
     out = {"sel": N - 1, "pts": N - 1, "jac_bytes": 28 * N - 1}
     for lq in range(1, L):
