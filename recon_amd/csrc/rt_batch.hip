@@ -1051,9 +1051,10 @@ struct rt_batch {
     // pinned staging of the host tables each prepare / retrack uploads (the
     // copies are queued, so their sources must outlive the call)
     HostStage st_desc, st_calib, st_order, st_trel;
+    HostStage st_res;  // page-locked landing buffer of the results (poses, rms, status)
     // last batch
     int n = 0;
-    std::vector<PairDesc> desc;
+    std::vector<PairDesc> desc, desc_next;  // the prepared pairs; the next preparation's, swapped in on success
     int maxW = 0, maxH = 0, maxN = 0;
     // tuning and graphs
     std::map<std::string, std::pair<int, int>> tuned;  // key -> (variant, grid)
@@ -1149,7 +1150,7 @@ extern "C" void rt_batch_destroy(rt_batch* b) {
     if (b->graph_sel) hipGraphExecDestroy(b->graph_sel);
     if (b->graph_track) hipGraphExecDestroy(b->graph_track);
     for (void* p : b->allocs) hipFree(p);
-    b->st_desc.release(); b->st_calib.release(); b->st_order.release(); b->st_trel.release();
+    b->st_desc.release(); b->st_calib.release(); b->st_order.release(); b->st_trel.release(); b->st_res.release();
     if (b->inputs) hipEventDestroy(b->inputs);
     if (b->proposed) hipEventDestroy(b->proposed);
     if (b->pyramid) hipEventDestroy(b->pyramid);
@@ -1344,11 +1345,15 @@ static int capture(rt_batch* b, hipGraphExec_t* out, int prep) {
 }
 
 static int fetch_results(rt_batch* b, rt_pair_result* out) {
-    std::vector<float> tr(16 * (size_t)b->n), rms(b->n);
-    std::vector<int> st(b->n);
-    if (hipMemcpyAsync(tr.data(), b->trel_out, tr.size() * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
-        hipMemcpyAsync(rms.data(), b->rms, rms.size() * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
-        hipMemcpyAsync(st.data(), b->status, st.size() * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+    // into page-locked memory: direct copies, no staging inside the runtime
+    const size_t n = b->n;
+    float* tr = (float*)b->st_res.get(24 * n * 4);
+    if (!tr) return set_error(RT_ERR_HIP, "result buffer");
+    float* rms = tr + 16 * n;
+    int* st = (int*)(rms + n);
+    if (hipMemcpyAsync(tr, b->trel_out, 16 * n * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipMemcpyAsync(rms, b->rms, n * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
+        hipMemcpyAsync(st, b->status, n * 4, hipMemcpyDeviceToHost, b->stream) != hipSuccess ||
         hipStreamSynchronize(b->stream) != hipSuccess)
         return set_error(RT_ERR_HIP, "result copy");
     for (int i = 0; i < b->n; i++) {
@@ -1443,23 +1448,39 @@ static int build_batch(const rt_params& prm, const BatchMem& m, int n, const rt_
         maxW = std::max(maxW, d.W); maxH = std::max(maxH, d.H); maxN = std::max(maxN, d.N);
     }
     // work queue order: decreasing cost (points x iterations), ties by index
-    for (int i = 0; i < n; i++) order[i] = i;
-    auto cost = [&](int i) { long c = 0; for (int k = 0; k < desc[i].L; k++) c += desc[i].iters[k]; return c * desc[i].N + (long)desc[i].W * desc[i].H / 8; };
-    std::stable_sort(order, order + n, [&](int x, int y) { return cost(x) > cost(y); });
+    // (each pair's cost once; a batch of equal costs keeps index order, as
+    // the stable sort would)
+    std::vector<long> cost(n);
+    std::vector<int> ord(n);
+    bool uniform = true;
+    for (int i = 0; i < n; i++) {
+        long c = 0;
+        for (int k = 0; k < desc[i].L; k++) c += desc[i].iters[k];
+        cost[i] = c * desc[i].N + (long)desc[i].W * desc[i].H / 8;
+        uniform = uniform && cost[i] == cost[0];
+        ord[i] = i;
+    }
+    if (!uniform) std::stable_sort(ord.begin(), ord.end(), [&](int x, int y) { return cost[x] > cost[y]; });
+    memcpy(order, ord.data(), 4 * (size_t)n);  // order may be page-locked staging memory: written once
     return RT_OK;
 }
 
 extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
     if (!b || !pairs || n < 1 || n > b->max_pairs) return set_error(RT_ERR_ARG, "rt_batch_prepare: bad argument");
     if (hipSetDevice(b->device) != hipSuccess) return set_error(RT_ERR_HIP, "hipSetDevice");
-    std::vector<PairDesc> desc(n);
-    std::vector<float> calib(96 * (size_t)n);
-    std::vector<float> trel(16 * (size_t)n);
-    std::vector<int> order(n);
+    // descriptors into a reused host vector; calibration blocks, initial poses
+    // and the work order straight into the pinned staging buffers they are
+    // uploaded from (no zero-filled temporaries, no second copies)
+    std::vector<PairDesc>& desc = b->desc_next;
+    desc.resize(n);
+    float* calib = (float*)b->st_calib.get(96 * 4 * (size_t)n);
+    float* trel = (float*)b->st_trel.get(64 * (size_t)n);
+    int* order = (int*)b->st_order.get(4 * (size_t)n);
+    if (!calib || !trel || !order) return set_error(RT_ERR_HIP, "rt_batch_prepare: staging memory");
     const BatchMem m{b->arena, b->pair_bytes, b->max_w, b->max_h, (size_t)b->prm.n_select, b->prm.mode != RT_PHOTOMETRIC,
                      b->calib_dev, b->trel_in, b->trel_out, b->rms, b->status};
     int maxW = 0, maxH = 0, maxN = 0;
-    int rc = build_batch(b->prm, m, n, pairs, desc.data(), calib.data(), trel.data(), order.data(), maxW, maxH, maxN);
+    int rc = build_batch(b->prm, m, n, pairs, desc.data(), calib, trel, order, maxW, maxH, maxN);
     if (rc) return rc;
     // the network's inputs in the preparation: pairs of one size, a multiple
     // of 8, with the current frame 2-byte (gray) and 4-byte (depth) aligned
@@ -1471,12 +1492,12 @@ extern "C" int rt_batch_prepare(rt_batch* b, int n, const rt_pair* pairs) {
         desc[i].feat = feat ? b->feat + (size_t)i * (desc[i].W / 8) * (desc[i].H / 8) * 4 : nullptr;
     b->feat_ready = feat;
     if (!b->st_desc.upload(b->desc_dev, desc.data(), sizeof(PairDesc) * n, b->stream) ||
-        !b->st_calib.upload(b->calib_dev, calib.data(), calib.size() * 4, b->stream) ||
-        !b->st_order.upload(b->order_dev, order.data(), 4 * (size_t)n, b->stream) ||
-        !b->st_trel.upload(b->trel_in, trel.data(), 64 * (size_t)n, b->stream))
+        !b->st_calib.send(b->calib_dev, 96 * 4 * (size_t)n, b->stream) ||
+        !b->st_order.send(b->order_dev, 4 * (size_t)n, b->stream) ||
+        !b->st_trel.send(b->trel_in, 64 * (size_t)n, b->stream))
         return set_error(RT_ERR_HIP, "descriptor upload");
     bool changed = b->n != n || b->maxW != maxW || b->maxH != maxH || b->maxN != maxN;
-    b->n = n; b->desc = desc; b->maxW = maxW; b->maxH = maxH; b->maxN = maxN;
+    b->n = n; b->desc.swap(desc); b->maxW = maxW; b->maxH = maxH; b->maxN = maxN;
     if (changed || !b->graph_pyr || !b->graph_bins || !b->graph_sel) {
         if ((rc = capture(b, &b->graph_pyr, 1)) || (rc = capture(b, &b->graph_bins, 2)) || (rc = capture(b, &b->graph_sel, 3)))
             return rc;

@@ -47,7 +47,11 @@ struct HostStage {
         void* h = get(bytes);
         if (!h) return false;
         memcpy(h, src, bytes);
-        return hipMemcpyAsync(dst, h, bytes, hipMemcpyHostToDevice, s) == hipSuccess && queued(s);
+        return send(dst, bytes, s);
+    }
+    // copy the first `bytes` of the stage, written in place after get(), to `dst` on `s`
+    bool send(void* dst, size_t bytes, hipStream_t s) {
+        return hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, s) == hipSuccess && queued(s);
     }
     void release() {
         if (pending) hipEventSynchronize(ev);
