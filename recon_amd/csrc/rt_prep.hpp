@@ -28,13 +28,13 @@ __device__ __forceinline__ void convert_pyramid_block(int bx, int by, const uint
             size_t o = (size_t)y * W + x0;
             if (SRC == 3) {  // 8-bit gray only: level 0 stays in its input form
                 uchar4 gg = *reinterpret_cast<const uchar4*>(g8 + o);
-                v[r][0] = (float)gg.x / 255.0f; v[r][1] = (float)gg.y / 255.0f;
-                v[r][2] = (float)gg.z / 255.0f; v[r][3] = (float)gg.w / 255.0f;
+                v[r][0] = u8_unit(gg.x); v[r][1] = u8_unit(gg.y);
+                v[r][2] = u8_unit(gg.z); v[r][3] = u8_unit(gg.w);
             } else if (SRC == 0) {
                 uchar4 gg = *reinterpret_cast<const uchar4*>(g8 + o);
                 ushort4 dd = *reinterpret_cast<const ushort4*>(d16 + o);
-                v[r][0] = (float)gg.x / 255.0f; v[r][1] = (float)gg.y / 255.0f;
-                v[r][2] = (float)gg.z / 255.0f; v[r][3] = (float)gg.w / 255.0f;
+                v[r][0] = u8_unit(gg.x); v[r][1] = u8_unit(gg.y);
+                v[r][2] = u8_unit(gg.z); v[r][3] = u8_unit(gg.w);
                 *reinterpret_cast<float4*>(g0 + o) = make_float4(v[r][0], v[r][1], v[r][2], v[r][3]);
                 *reinterpret_cast<float4*>(z + o) =
                     make_float4(-(float)dd.x, -(float)dd.y, -(float)dd.z, -(float)dd.w);

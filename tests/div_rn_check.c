@@ -3,7 +3,8 @@
  * IEEE division: random normal operands over 56 binades, and the
  * back-projection domain of the gradient pass, a = (x - c) z over pixel
  * columns x and every 16-bit depth z, divided by the focal length and then by
- * z. Prints the mismatch count; exit status 1 on any mismatch. */
+ * z, and the 256 8-bit gray values over 255. Prints the mismatch count;
+ * exit status 1 on any mismatch. */
 #include <math.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -45,6 +46,8 @@ int main(int argc, char** argv) {
                 n++; bad += !same(div_rn(p, z, 1.0f / z), p / z);
             }
     }
+    /* the 8-bit gray values over 255 (the pyramid's conversion, u8_unit) */
+    for (int t = 0; t < 256; t++) { n++; bad += !same(div_rn((float)t, 255.0f, 1.0f / 255.0f), (float)t / 255.0f); }
     printf("%ld mismatches in %ld divisions\n", bad, n);
     return bad ? 1 : 0;
 }
