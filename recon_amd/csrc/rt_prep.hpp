@@ -675,7 +675,11 @@ __device__ __forceinline__ void photo_point(float4 p4, float iref, const float* 
     f3 p3{p4.x, p4.y, p4.z};
     f3 r3ir = mul4(T, p3);
     f3 r3 = SIMPLE ? r3ir : mul4(c + TLR, r3ir);
-    f2 pu{r3.x / r3.z, r3.y / r3.z};
+    // the divisions as div_rn (the IEEE quotients; a zero or non-finite z gives
+    // NaN where the division gives an infinity, which the finiteness test
+    // below rejects alike)
+    const float iz = 1.0f / r3.z;
+    f2 pu{div_rn(r3.x, r3.z, iz), div_rn(r3.y, r3.z, iz)};
     f2 p2 = SIMPLE ? project_simple(pu, c + KR) : distort(pu, c + KCR, c + KR);
     f2 p{a * p2.x + b, a * p2.y + b};
     ir = 1.0f;
@@ -693,7 +697,8 @@ __device__ __forceinline__ void photo_point(float4 p4, float iref, const float* 
             pl2 = f3{p2.x, p2.y, 1.0f};
         } else {
             f3 rl3 = mul4(c + TRL, r3);
-            rl3.x /= rl3.z; rl3.y /= rl3.z; rl3.z = 1.0f;
+            const float il = 1.0f / rl3.z;
+            rl3.x = div_rn(rl3.x, rl3.z, il); rl3.y = div_rn(rl3.y, rl3.z, il); rl3.z = 1.0f;
             pl2 = mul3(c + KL, rl3);
         }
         bool fin2 = fabsf(pl2.x) < 1e9f && fabsf(pl2.y) < 1e9f;
@@ -702,7 +707,7 @@ __device__ __forceinline__ void photo_point(float4 p4, float iref, const float* 
             float zc = zcur[xdi2 + (size_t)ydi2 * W];
             float zerr = zc - r3.z;
             zerr *= zerr;
-            if (zerr < 10000.0f) zW = 1.0f - zerr / 10000.0f;
+            if (zerr < 10000.0f) zW = 1.0f - div_rn(zerr, 10000.0f, 1.0f / 10000.0f);
         }
     }
 }
@@ -713,13 +718,16 @@ __device__ __forceinline__ float mad_scale(int m, int bins) {
 }
 // generateWeights64Kernel (histogram64.cu:79-94): Tukey (1-(u/4.6851)^2) zW
 // inside the threshold, or Huber sqrt(min(1, 1.345/u)) zW.
+// The divisions by s (> 0: the MAD scale is at least 1.4826 * 3 / 255) and by
+// the Tukey constant run as div_rn (the same quotients); 1 / s is the same for
+// every point of an iteration.
 __device__ __forceinline__ float robust_weight(float r, float zw, float s, int robust) {
-    float u = fabsf(r) / s;
+    float u = div_rn(fabsf(r), s, 1.0f / s);
     float w = 0.0f;
     if (robust == 0) {
-        float bb = 4.6851f;
+        const float bb = 4.6851f;
         if (u <= bb) {
-            float ub = u / bb, ub2 = ub * ub;
+            float ub = div_rn(u, bb, 1.0f / bb), ub2 = ub * ub;
             w = (1.0f - ub2) * zw;
         }
     } else {
