@@ -282,9 +282,9 @@ int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, void* host, 
  * points and the point records of levels 0-3, the pair's arena slot start
  * and the slot size, then in the geometric modes (else -1) the offsets of the
  * keyframe cloud, its normals and the current cloud of levels 0-3, of the ICP
- * selection, its squared residuals, the ICP counts and variances, then 3
- * zeros. order: the work order (n ints); maxes: the largest width, height
- * and point count. */
+ * selection, its squared residuals, the ICP counts and variances, the
+ * offset of the current depth map in 8x8-pixel blocks, then 2 zeros. order:
+ * the work order (n ints); maxes: the largest width, height and point count. */
 int rt_batch_layout(const rt_params* params, int max_width, int max_height, int n, const rt_pair* pairs,
                     int64_t* fields, int* order, int maxes[3]);
 
@@ -293,7 +293,7 @@ int rt_batch_layout(const rt_params* params, int max_width, int max_height, int 
  * forbid hipMalloc): rt_batch_track without a batch object. The kernels'
  * working memory (per-pair pyramids, gradient bins, histograms, selections,
  * point records, descriptor and pose tables, the work queue) is one fixed
- * device array of RT_NOALLOC_SCRATCH_BYTES (16.9 MiB) that each GPU the library
+ * device array of RT_NOALLOC_SCRATCH_BYTES (21.6 MiB) that each GPU the library
  * runs on holds until the process ends; the call allocates nothing. The caller
  * passes the frames (device pointers), calibration and initial poses as in
  * rt_pair, and the results; work runs on `stream` (a hipStream_t, NULL for the
@@ -307,7 +307,7 @@ int rt_batch_layout(const rt_params* params, int max_width, int max_height, int 
 #define RT_NOALLOC_MAX_WIDTH 640
 #define RT_NOALLOC_MAX_HEIGHT 480
 #define RT_NOALLOC_MAX_POINTS 8192
-#define RT_NOALLOC_SCRATCH_BYTES 17683456
+#define RT_NOALLOC_SCRATCH_BYTES 22598656
 int rt_batch_track_noalloc(int device, int n, const rt_pair* pairs, const rt_params* params, void* stream,
                            rt_pair_result* out);
 

@@ -39,6 +39,7 @@ struct PairDesc {
     int simple;             // calib_simple(calib), evaluated at preparation
     float ifx, ify;         // RN(1 / fx), RN(1 / fy) of the depth camera (IEEE divisions on the host)
     bf16* feat;             // the pose network's inputs of the pair (cells of 8 x 8 pixels, 4 channels), or null
+    uint16_t* zblk;         // the current depth map in 8 x 8-pixel blocks of 64 values (one 128-byte line each)
     const uint8_t* g8[2];
     const uint16_t* d16[2];
     float* gray[2][4];
@@ -84,32 +85,59 @@ __device__ __forceinline__ void load_lut(float* lut) {
 
 // pyramid levels 1..L-1 of one frame (fastDownSampleHdrKernel, downSample2Cuda.cu:35-42);
 // the keyframe's (which 0) are built by kb_bins from its gray tile
-// With d.feat set (which 1), also the pose network's inputs of the current
-// frame over the block's 8 x 8 cells: four threads per cell, two columns
-// each, column sums over the rows in order, then pairwise: the order of
-// cell8_features, the network's input builder
+// The current frame (which 1) also gets its depth map copied into 8 x 8-pixel
+// blocks for the tracker, and with d.feat set the pose network's inputs over
+// the block's 8 x 8 cells: four threads per cell, two columns each, column
+// sums over the rows in order, then pairwise (the order of cell8_features,
+// the network's input builder)
 __global__ __launch_bounds__(256) void kb_pyramid(const PairDesc* __restrict__ P, int which) {
     const PairDesc& d = P[blockIdx.z];
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 64 >= d.H) return;
     convert_pyramid_block<3>(blockIdx.x, blockIdx.y, d.g8[which], nullptr, nullptr, d.W, d.H, d.L, d.calib, nullptr,
                              d.gray[which][1], d.gray[which][2], d.gray[which][3], nullptr);
+    if (which != 1) return;
     bf16* const feat = d.feat;
-    if (which != 1 || !feat) return;
     const int W = d.W, H = d.H;
     const int cell = threadIdx.x >> 2, part = threadIdx.x & 3;
     const int cx = blockIdx.x * 8 + (cell & 7), cy = blockIdx.y * 8 + (cell >> 3);
-    const bool valid = cx < (W >> 3) && cy < (H >> 3);
-    // clamped into the image; what an invalid cell sums is never written
-    const unsigned px = min(cx * 8 + 2 * part, W - 2), py = min(cy * 8, H - 8);
+    // columns clamped into the image (W is a multiple of 8), rows one by one;
+    // what lies outside is neither stored nor summed into a written cell
+    const unsigned px = min(cx * 8 + 2 * part, W - 2);
     const RT_GLOBAL uint8_t* g8 = gmem(d.g8[1]);
     const RT_GLOBAL uint16_t* d16 = gmem(d.d16[1]);
+    const bool d4 = ((uintptr_t)d.d16[1] & 3) == 0;  // two depths per 4-byte load
     unsigned gw[8], dw[8];
 #pragma unroll
     for (int r = 0; r < 8; r++) {
-        const unsigned o = px + __umul24(py + r, W);
-        gw[r] = at_offset((const RT_GLOBAL uint16_t*)g8, o);    // two gray bytes (o even)
-        dw[r] = at_offset((const RT_GLOBAL unsigned*)d16, 2 * o);  // two depths
+        const unsigned o = px + __umul24(min(cy * 8 + r, H - 1), W);
+        if (feat) gw[r] = at_offset((const RT_GLOBAL uint16_t*)g8, o);  // two gray bytes (o even)
+        dw[r] = d4 ? at_offset((const RT_GLOBAL unsigned*)d16, 2 * o)
+                   : (unsigned)d16[o] | (unsigned)d16[o + 1] << 16;
     }
+    // the current depth map in 8 x 8-pixel blocks (the tracker's depth taps):
+    // block (cx, cy) is 32 words, row r of it words 4r .. 4r + 3, two depths
+    // each. The region's 64 blocks are assembled in LDS, then each row of 8
+    // blocks (consecutive in memory) is stored as one 1 KB run, 32 bytes per
+    // thread. Rows past H (when H is not a multiple of 8) hold copies of row
+    // H - 1 and are never read.
+    __shared__ __align__(16) unsigned zt[64 * 32];
+#pragma unroll
+    for (int r = 0; r < 8; r++) zt[cell * 32 + 4 * r + part] = dw[r];
+    __syncthreads();
+    {
+        const int W8 = W >> 3, H8 = (H + 7) >> 3;
+        const int cyl = threadIdx.x >> 5, w = (threadIdx.x & 31) * 8;  // cell row, first of 8 words
+        const int bcx = blockIdx.x * 8 + (w >> 5), bcy = blockIdx.y * 8 + cyl;
+        if (bcx < W8 && bcy < H8) {
+            typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+            const u32x4* src = (const u32x4*)&zt[cyl * 256 + w];
+            RT_GLOBAL u32x4* dst = (RT_GLOBAL u32x4*)((RT_GLOBAL unsigned*)gmem(d.zblk) + 32 * ((size_t)bcy * W8 + bcx) + (w & 31));
+            dst[0] = src[0];
+            dst[1] = src[1];
+        }
+    }
+    if (!feat) return;
+    const bool valid = cx < (W >> 3) && cy < (H >> 3);
     unsigned fg = 0, fn = 0;
     float c0 = 0.0f, c1 = 0.0f;
 #pragma unroll
@@ -841,7 +869,7 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
                         p4 = ldg(d.pts, i);
                         ir = ldg(jv.iref, i);
                     }
-                    const U16DepthSrc zs{d.d16[1]};
+                    const U16BlockDepthSrc zs{gmem(d.zblk)};  // built by kb_pyramid
                     if (layer == 0) {
                         const U8Src g{d.g8[1], lut};
                         if (simple) photo_point<true>(p4, ir, Tl, a, b, g, wl, hl, zs, W, H, cl, rr, zz);
@@ -1084,6 +1112,7 @@ static size_t pair_arena_bytes(int max_w, int max_h, size_t N, bool icp = false)
     size_t per = 0;  // inputs are borrowed device pointers; the arena holds derived data only
     per += 2 * align256(npx * 4 / 3 + 64);      // two gray pyramids, levels 1..3
     per += align256(npx);                       // gradient-magnitude bins
+    per += align256((size_t)max_w * ((max_h + 7) & ~7) * 2);  // current depth map in 8 x 8 blocks
     per += align256(256 * 4) + align256(N * 4) + align256(N * 16) + 4 * align256(jac_bytes((int)N));
     if (icp) {
         for (int l = 0; l < 4; l++) per += 3 * align256((npx >> (2 * l)) * 16);  // rpos, rnrm, cpos
@@ -1422,6 +1451,7 @@ static int build_batch(const rt_params& prm, const BatchMem& m, int n, const rt_
         }
         d.z[0] = d.z[1] = nullptr;   // depth is read from the 16-bit input
         d.bins = (uint8_t*)(base + off); off += align256((size_t)m.max_w * m.max_h);
+        d.zblk = (uint16_t*)(base + off); off += align256((size_t)m.max_w * ((m.max_h + 7) & ~7) * 2);
         d.ghist = (unsigned*)(base + off); off += align256(256 * 4);
         const size_t NN = m.nsel;
         d.sel = (int*)(base + off); off += align256(NN * 4);
@@ -1641,7 +1671,8 @@ extern "C" int rt_debug_depth_reciprocals(int device, float* host, int n) {
 namespace rt {
 constexpr size_t ca256(size_t x) { return (x + 255) & ~(size_t)255; }
 constexpr size_t NA_PAIR_BYTES = 2 * ca256((size_t)RT_NOALLOC_MAX_WIDTH * RT_NOALLOC_MAX_HEIGHT * 4 / 3 + 64) +
-                                 ca256((size_t)RT_NOALLOC_MAX_WIDTH * RT_NOALLOC_MAX_HEIGHT) + ca256(256 * 4) +
+                                 ca256((size_t)RT_NOALLOC_MAX_WIDTH * RT_NOALLOC_MAX_HEIGHT) +
+                                 ca256((size_t)RT_NOALLOC_MAX_WIDTH * ((RT_NOALLOC_MAX_HEIGHT + 7) & ~7) * 2) + ca256(256 * 4) +
                                  ca256((size_t)RT_NOALLOC_MAX_POINTS * 4) + ca256((size_t)RT_NOALLOC_MAX_POINTS * 16) +
                                  4 * ca256((size_t)RT_NOALLOC_MAX_POINTS * 28);
 // table region: descriptors, work order, queue, calibration blocks, poses, rms, status
@@ -1804,7 +1835,8 @@ extern "C" int rt_batch_layout(const rt_params* params, int max_w, int max_h, in
         }
         f[33] = icp ? off(d.isel) : -1; f[34] = icp ? off(d.ir2) : -1;
         f[35] = icp ? off(d.icnt) : -1; f[36] = icp ? off(d.ivar) : -1;
-        f[37] = f[38] = f[39] = 0;
+        f[37] = off(d.zblk);
+        f[38] = f[39] = 0;
     }
     return RT_OK;
 }

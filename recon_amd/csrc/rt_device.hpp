@@ -142,6 +142,20 @@ struct U16DepthSrc {
     __device__ U16DepthSrc(const uint16_t* q) : p(gmem(q)) {}
     __device__ __forceinline__ float operator[](size_t i) const { return -(float)p[i]; }
 };
+// A 16-bit depth map stored in 8 x 8-pixel blocks of 64 values, block
+// (x / 8, y / 8) at (y / 8) (W / 8) + x / 8, row-major inside: one 128-byte
+// line per block, so that nearby taps share lines in both directions. The same
+// negated values as U16DepthSrc.
+struct U16BlockDepthSrc {
+    const RT_GLOBAL uint16_t* __restrict__ p;
+};
+// the depth sample at pixel (x, y) of a W-wide depth source
+template <class ZS>
+__device__ __forceinline__ float depth_tap(const ZS& z, int x, int y, int W) { return z[x + (size_t)y * W]; }
+__device__ __forceinline__ float depth_tap(const U16BlockDepthSrc& z, int x, int y, int W) {
+    const unsigned blk = __umul24((unsigned)y >> 3, (unsigned)W >> 3) + ((unsigned)x >> 3);
+    return -(float)z.p[(blk << 6) + ((y & 7) << 3) + (x & 7)];
+}
 
 // bilinearInterpolation(xdi,ydi,fx,fy,...) (kernelUtils.h:267-272)
 __device__ __forceinline__ float blend(float v0, float v1, float v2, float v3, float fx, float fy) {

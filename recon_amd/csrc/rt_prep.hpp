@@ -704,7 +704,7 @@ __device__ __forceinline__ void photo_point(float4 p4, float iref, const float* 
         bool fin2 = fabsf(pl2.x) < 1e9f && fabsf(pl2.y) < 1e9f;
         int xdi2 = fin2 ? (int)(pl2.x + 0.5f) : -1, ydi2 = fin2 ? (int)(pl2.y + 0.5f) : -1;
         if (xdi2 >= 0 && ydi2 >= 0 && xdi2 < W && ydi2 < H) {
-            float zc = zcur[xdi2 + (size_t)ydi2 * W];
+            float zc = depth_tap(zcur, xdi2, ydi2, W);
             float zerr = zc - r3.z;
             zerr *= zerr;
             if (zerr < 10000.0f) zW = 1.0f - div_rn(zerr, 10000.0f, 1.0f / 10000.0f);

@@ -65,6 +65,7 @@ def _buffers(p, max_w, max_h):
             else:
                 assert o == -1
     b["bins"] = (int(p[11]), W * H)
+    b["zblk"] = (int(p[37]), W * ((H + 7) // 8 * 8) * 2)
     b["ghist"] = (int(p[12]), 256 * 4)
     b["sel"] = (int(p[13]), N * 4)
     b["pts"] = (int(p[14]), N * 16)
@@ -89,8 +90,8 @@ def _check_layout(f, order, maxes, max_w, max_h, N, icp=False):
     n = len(f)
     slot = int(f[0, 20])
     npx = max_w * max_h
-    want = 2 * _align(npx * 4 // 3 + 64) + _align(npx) + _align(1024) + _align(N * 4) + _align(N * 16) + \
-        4 * _align(N * 28)
+    want = 2 * _align(npx * 4 // 3 + 64) + _align(npx) + _align(max_w * ((max_h + 7) // 8 * 8) * 2) + \
+        _align(1024) + _align(N * 4) + _align(N * 16) + 4 * _align(N * 28)
     if icp:
         want += sum(3 * _align((npx >> (2 * l)) * 16) for l in range(4)) + 2 * _align(npx * 4) + _align(64)
     assert slot == want
@@ -124,16 +125,29 @@ def _kb_pyramid(W, H, L, maxW, maxH):
         ev = ((t & 15)[inside] % 2 == 0) & ((t >> 4)[inside] % 2 == 0)
         out["gray_3"] = ((y0[ev] // 8) * (W >> 3) + x0[ev] // 8).max()
         assert ((t & 15)[inside][ev] + 1).max() <= 15 and ((t >> 4)[inside][ev] + 1).max() <= 15  # l2tile[16][17]
+    # the current frame's depth copy into 8x8 blocks and the network's inputs:
+    # 4 threads per 8x8 cell, two columns each, columns clamped to px <= W - 2,
+    # rows one by one to <= H - 1; block rows stored while below H
+    assert W % 8 == 0
+    cell, part = t[live] >> 2, t[live] & 3
+    cx = bx[live] * 8 + (cell & 7)
+    cy = by[live] * 8 + (cell >> 3)
+    px = np.minimum(cx * 8 + 2 * part, W - 2)
+    rows = np.minimum(cy[:, None] * 8 + np.arange(8)[None, :], H - 1)
+    out["g8_feat"] = (rows.max(axis=1) * W + px + 1).max()        # two gray bytes
+    out["d16_feat"] = (rows.max(axis=1) * W + px + 1).max()       # two depths
+    # LDS assembly: word cell * 32 + 4r + part of 64 * 32; stores: thread t writes
+    # words w .. w + 7 (w = 8 (t % 32)) of cell row t // 32, as two 16-byte stores
+    lds = (t[live] >> 2)[:, None] * 32 + 4 * np.arange(8)[None, :] + (t[live] & 3)[:, None]
+    assert lds.max() < 64 * 32
+    tt = t[live]
+    w = (tt & 31) * 8
+    bcx, bcy = bx[live] * 8 + (w >> 5), by[live] * 8 + (tt >> 5)
+    st = (bcx < W // 8) & (bcy < (H + 7) // 8)
+    words = 32 * (bcy * (W // 8) + bcx) + (w & 31) + 7
+    assert np.all(((32 * (bcy * (W // 8) + bcx) + (w & 31)) * 4) % 16 == 0)   # 16-byte aligned stores
+    out["zblk_words"] = words[st].max()
     if W % 8 == 0 and H % 8 == 0:
-        # the network's inputs of the current frame: 4 threads per 8x8 cell, two
-        # columns each, loads clamped to px <= W - 2, py <= H - 8; rows py .. py + 7
-        cell, part = t[live] >> 2, t[live] & 3
-        cx = bx[live] * 8 + (cell & 7)
-        cy = by[live] * 8 + (cell >> 3)
-        px = np.minimum(cx * 8 + 2 * part, W - 2)
-        py = np.minimum(cy * 8, H - 8)
-        out["g8_feat"] = ((py + 7) * W + px + 1).max()                # two gray bytes
-        out["d16_feat"] = ((py + 7) * W + px + 1).max()               # two depths
         ok = (cx < W // 8) & (cy < H // 8)
         out["feat"] = (4 * (cy[ok] * (W // 8) + cx[ok]) + 3).max()    # channels 2, 3
     return out
@@ -224,7 +238,12 @@ def _kb_track(W, H, L, N, maxN, pc):
         # bilinear taps at xdi .. xdi + 1, ydi .. ydi + 1 with 0 <= xdi < wl - 1, 0 <= ydi < hl - 1
         out[f"img_{layer}"] = (wl - 2) + (hl - 2) * wl + 1 + wl
         assert out[f"img_{layer}"] <= wl * hl - 1
-    out["depth_tap"] = (W - 1) + (H - 1) * W
+    # the depth tap reads the current depth map in 8x8 blocks (kb_pyramid's copy):
+    # pixel (x, y) at ((y // 8) * (W // 8) + x // 8) * 64 + (y % 8) * 8 + x % 8, 0 <= x < W, 0 <= y < H
+    x, y = np.meshgrid(np.arange(W), np.arange(H), indexing="ij")
+    blk = ((y // 8) * (W // 8) + x // 8) * 64 + (y % 8) * 8 + x % 8
+    out["zblk_tap"] = blk.max()
+    assert np.unique(blk).size == W * H                  # a distinct element per pixel
     assert N <= maxN                                     # LDS residuals: maxN entries
     assert min(N, pc) <= min(maxN, pc)                   # point cache: min(maxN, capacity) entries
     return out
@@ -262,7 +281,8 @@ def _check_kernels(f, maxes, max_w, max_h, pcs=(0, 5120, 5632, 8192)):
         bufsz = {"g8": W * H, "d16": W * H, "bins": W * H, "sel": N, "pts": N}
         for l in range(1, L):
             bufsz[f"gray_{l}"] = (W >> l) * (H >> l)
-        bufsz.update(g8_feat=W * H, d16_feat=W * H, feat=4 * (W // 8) * (H // 8))
+        bufsz.update(g8_feat=W * H, d16_feat=W * H, feat=4 * (W // 8) * (H // 8),
+                     zblk_words=W * ((H + 7) // 8 * 8) // 2)
         pyr = _kb_pyramid(W, H, L, maxW, maxH)
         for k, v in pyr.items():
             assert v < bufsz[k], (p, "kb_pyramid", k, v)
@@ -281,7 +301,7 @@ def _check_kernels(f, maxes, max_w, max_h, pcs=(0, 5120, 5632, 8192)):
         assert ref["jac_bytes"] < 28 * N
         for pc in pcs:
             tr = _kb_track(W, H, L, N, maxN, pc)
-            assert tr["depth_tap"] < W * H
+            assert tr["zblk_tap"] < W * ((H + 7) // 8 * 8)
             for layer in range(1, L):
                 assert tr[f"img_{layer}"] < bufsz[f"gray_{layer}"]
             assert tr["img_0"] < W * H
