@@ -267,7 +267,7 @@ void* rt_batch_stream(rt_batch* b);
 int rt_batch_variant_info(int v, int info[6]);
 /* Inspection of one pair of the last prepared batch (for parity tests):
  * RT_DBG_GRAD_HIST, RT_DBG_SELECTED, RT_DBG_POINTS, RT_DBG_JACOBIAN (level),
- * RT_DBG_TREL (the pair's initial pose); in the geometric and bi-objective
+ * RT_DBG_TREL (the pair's initial pose), RT_DBG_DEPTH_BLOCKS; in the geometric and bi-objective
  * modes also RT_DBG_REF_CLOUD, RT_DBG_CUR_CLOUD (level) and, after tracking,
  * RT_DBG_ICP_COUNT and RT_DBG_ICP_VAR (level) of the last tracking launch;
  * layouts as for single trackers. */
@@ -335,7 +335,9 @@ enum {
     RT_DBG_SYSTEM = 17,      /* double 54: photometric terms then geometric terms, 27 each:
                                  A upper triangle row by row (21), then b (6) */
     RT_DBG_KF_WEIGHT = 18,   /* float W*H fusion weights */
-    RT_DBG_MAD_HIST = 19     /* uint32 mad_bins (last photometric iteration) */
+    RT_DBG_MAD_HIST = 19,    /* uint32 mad_bins (last photometric iteration) */
+    RT_DBG_DEPTH_BLOCKS = 20 /* batches: uint16 W*ceil8(H), the current depth map in 8x8-pixel blocks:
+                                pixel (x, y) at ((y/8)*(W/8) + x/8)*64 + (y%8)*8 + x%8 */
 };
 int rt_debug_read(rt_tracker* t, int what, int level, void* host, size_t bytes);
 /* Set the device relative pose (TrelDev) directly. */

@@ -1597,6 +1597,7 @@ extern "C" int rt_batch_debug_read(rt_batch* b, int pair, int what, int level, v
             if (level < 0 || level >= d.L) return set_error(RT_ERR_ARG, "bad level");
             src = d.jac[level]; n = jac_bytes(d.N); break;
         case RT_DBG_TREL: src = d.trel_in; n = 64; break;
+        case RT_DBG_DEPTH_BLOCKS: src = d.zblk; n = (size_t)d.W * ((d.H + 7) & ~7) * 2; break;
         case RT_DBG_REF_CLOUD:
         case RT_DBG_CUR_CLOUD:
         case RT_DBG_ICP_COUNT:

@@ -11,7 +11,7 @@ from . import (BIOBJECTIVE, GEOMETRIC, PHOTOMETRIC, RtPair, RtPairResult, RtPara
 
 DBG = dict(REF_GRAY=0, CUR_GRAY=1, REF_Z=2, CUR_Z=3, GRAD_MAG=4, GRAD_HIST=5, SELECTED=6, POINTS=7, JACOBIAN=8,
            RESIDUAL=9, TREL=10, REF_CLOUD=11, CUR_CLOUD=12, ICP_SELECTED=13, ICP_COUNT=14, ICP_R2=15, ICP_VAR=16,
-           SYSTEM=17, KF_WEIGHT=18, MAD_HIST=19)
+           SYSTEM=17, KF_WEIGHT=18, MAD_HIST=19, DEPTH_BLOCKS=20)
 
 
 def default_params(**kw):

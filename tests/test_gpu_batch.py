@@ -150,3 +150,28 @@ def test_batch_kernel_variants_agree(N, iters, monkeypatch, capfd):
     assert (0, True) in kinds and (0, False) in kinds
     if N <= 8192:
         assert (1, True) in kinds and (1, False) in kinds
+
+
+def test_depth_blocks_are_the_current_depth_map():
+    """The tracker's depth taps read kb_pyramid's copy of the current depth
+    map in 8x8-pixel blocks: every pixel (x, y) must hold the input's value at
+    ((y // 8) * (W // 8) + x // 8) * 64 + (y % 8) * 8 + x % 8, also for a
+    height that is not a multiple of 8 (236 = 29 * 8 + 4, three levels)."""
+    print("case depth blocks 640x480 and 320x236", flush=True)
+    sizes = [(640, 480), (320, 236), (640, 480)]
+    pairs, host = [], []
+    for i, (W, H) in enumerate(sizes):
+        g0, d0, g1, d1, _ = synth.make_pair(60 + i, W, H, 3.0, 50.0)
+        bufs = [DeviceBuffer.from_array(a) for a in (g0, d0, g1, d1)]
+        host.append((d1, bufs))
+        pairs.append(dict(ref_gray=bufs[0].ptr, ref_depth=bufs[1].ptr, cur_gray=bufs[2].ptr, cur_depth=bufs[3].ptr,
+                          width=W, height=H, calib=synth.calib_block_np(W, H)))
+    bt = BatchTracker(len(pairs), 640, 480, mode=PHOTOMETRIC, n_select=2048, iters=(1, 1, 1, 1))
+    bt.prepare_only(bt.prepare(pairs))
+    for i, (W, H) in enumerate(sizes):
+        d1 = np.asarray(host[i][0]).reshape(H, W)
+        H8 = (H + 7) // 8 * 8
+        blk = bt.debug_read(i, "DEPTH_BLOCKS", 0, np.uint16, W * H8)
+        y, x = np.mgrid[0:H, 0:W]
+        idx = ((y // 8) * (W // 8) + x // 8) * 64 + (y % 8) * 8 + x % 8
+        assert np.array_equal(blk[idx], d1), f"pair {i} ({W}x{H})"
