@@ -93,8 +93,11 @@ __device__ __forceinline__ void load_lut(float* lut) {
 __global__ __launch_bounds__(256) void kb_pyramid(const PairDesc* __restrict__ P, int which) {
     const PairDesc& d = P[blockIdx.z];
     if ((int)blockIdx.x * 64 >= d.W || (int)blockIdx.y * 64 >= d.H) return;
+    __shared__ float lut[256];
+    load_lut(lut);
+    __syncthreads();
     convert_pyramid_block<3>(blockIdx.x, blockIdx.y, d.g8[which], nullptr, nullptr, d.W, d.H, d.L, d.calib, nullptr,
-                             d.gray[which][1], d.gray[which][2], d.gray[which][3], nullptr);
+                             d.gray[which][1], d.gray[which][2], d.gray[which][3], nullptr, lut);
     if (which != 1) return;
     bf16* const feat = d.feat;
     const int W = d.W, H = d.H;

@@ -13,7 +13,9 @@ __device__ __forceinline__ void convert_pyramid_block(int bx, int by, const uint
                                                          const float4* __restrict__ rgbd, int W, int H, int L,
                                                          const float* __restrict__ c, float* __restrict__ g0,
                                                          float* __restrict__ g1, float* __restrict__ g2,
-                                                         float* __restrict__ g3, float* __restrict__ z) {
+                                                         float* __restrict__ g3, float* __restrict__ z,
+                                                         const float* lut8 = nullptr) {
+    // lut8 (SRC 3): (float)t / 255 for t = 0..255 in LDS, the IEEE quotients
     __shared__ float l2tile[16][17];
     int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
     int x0 = (bx * 16 + tx) * 4, y0 = (by * 16 + ty) * 4;
@@ -28,8 +30,8 @@ __device__ __forceinline__ void convert_pyramid_block(int bx, int by, const uint
             size_t o = (size_t)y * W + x0;
             if (SRC == 3) {  // 8-bit gray only: level 0 stays in its input form
                 uchar4 gg = *reinterpret_cast<const uchar4*>(g8 + o);
-                v[r][0] = u8_unit(gg.x); v[r][1] = u8_unit(gg.y);
-                v[r][2] = u8_unit(gg.z); v[r][3] = u8_unit(gg.w);
+                v[r][0] = lut8[gg.x]; v[r][1] = lut8[gg.y];
+                v[r][2] = lut8[gg.z]; v[r][3] = lut8[gg.w];
             } else if (SRC == 0) {
                 uchar4 gg = *reinterpret_cast<const uchar4*>(g8 + o);
                 ushort4 dd = *reinterpret_cast<const ushort4*>(d16 + o);
