@@ -195,7 +195,7 @@ def main():
     ap.add_argument("--tmax", type=float, default=None, help="max translation (mm) of a pair")
     ap.add_argument("--weights", default=os.path.join(ROOT, "recon_amd", "data", "pose_cnn.npz"))
     ap.add_argument("--oracle-pairs", type=int, default=256,
-                    help="pairs judged against the FP64 oracle tracker started at the true motion")
+                    help="pairs checked against the FP64 oracle tracker started at the true motion")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="wall time of the CPU baseline sample (0: skip)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="CPU threads (0: nproc)")
     ap.add_argument("--seed", type=int, default=4)
@@ -399,7 +399,7 @@ def main():
         del lanes, bt2, net2
 
     # ---- convergence and errors against the FP64 oracle tracker started at the
-    # true motion (the judge), on the first S pairs of the batch
+    # true motion (the convergence reference), on the first S pairs of the batch
     S = max(P, min(args.oracle_pairs, B))
     t3 = time.perf_counter()
     ref64 = fp64_reference(g, d, gt, S, W, H, L, N, iters, cal, args.cpu_threads)
@@ -1015,7 +1015,7 @@ def mixed_bench(args):
 
 def fp64_reference(g, d, gt, n, W, H, L, N, iters, cal, threads):
     """The FP64 oracle tracker started at the true motion for the first n
-    pairs (the convergence judge): its converged poses, n x 4 x 4."""
+    pairs (the convergence reference): its converged poses, n x 4 x 4."""
     import threading
 
     import oracle
