@@ -937,15 +937,18 @@ __global__ __launch_bounds__(BS, WPE) void kb_track(const PairDesc* __restrict__
                     float4 ja = ldg(jv.a, i);
                     float2 jb = ldg(jv.b, i);
                     float jw[6] = {ja.x * w, ja.y * w, ja.z * w, ja.w * w, jb.x * w, jb.y * w};
+                    // each lane's running sums by fused multiply-adds (one rounding
+                    // per term: these sums are this kernel's own, in their own
+                    // order, and agree with the oracle's to rounding)
                     int q = 0;
 #pragma unroll
                     for (int r0 = 0; r0 < 6; r0++)
 #pragma unroll
-                        for (int c0 = r0; c0 < 6; c0++) acc[q++] += jw[r0] * jw[c0];
+                        for (int c0 = r0; c0 < 6; c0++) { acc[q] = __builtin_fmaf(jw[r0], jw[c0], acc[q]); q++; }
 #pragma unroll
-                    for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] += jw[r0] * wr;
-                    acc[27] += wr * wr;
-                    acc[28] += w * w;
+                    for (int r0 = 0; r0 < 6; r0++) acc[21 + r0] = __builtin_fmaf(jw[r0], wr, acc[21 + r0]);
+                    acc[27] = __builtin_fmaf(wr, wr, acc[27]);
+                    acc[28] = __builtin_fmaf(w, w, acc[28]);
                 };
                 if constexpr (RZ) {
 #pragma unroll PPT
